@@ -1,0 +1,205 @@
+/*
+ * phd_flux.h -- C ABI of the MI355X (gfx950) Voronoi face-flux library
+ * (libphd_flux.so).
+ *
+ * The library replaces the compiled inner loops of the reference's
+ * reconstruction / riemann / mesh / equation_state plugins.  Each entry
+ * point below names the reference method whose loop it replaces
+ * (path:line in rickyfernandez/phd-code).
+ *
+ * Conventions
+ *   - Every pointer in phd_cells / phd_faces and every array argument is a
+ *     DEVICE pointer, borrowed for the duration of the call.  The library
+ *     never frees or keeps them.
+ *   - Work is enqueued on the handle's HIP stream and returns immediately,
+ *     except the calls documented as synchronising (time step, error read).
+ *   - Fields follow the reference's named groups:
+ *       primitive    = density, velocity-x.., pressure
+ *                      (utils/particle_creator.pyx:50-52)
+ *       conservative = mass, momentum-x.., energy
+ *                      (utils/particle_creator.pyx:53-55)
+ *       gradient component g = n*dim + k is d prim_n / d x_k
+ *                      (reconstruction/reconstruction.pyx:470-480); gradients
+ *                      are stored cell-major: grad[cell*grad_ld + g].
+ *   - Tags: 0 Real, 1 Ghost.  Types: 2 Exterior (utils/particle_tags.py).
+ *   - Faces carry pair_i < pair_j with pair_i real, and per-cell face lists
+ *     (nbr_off/nbr_face) in ascending face id (mesh/tess.cpp:363-382,
+ *     mesh/tess3.cpp:509-534).
+ *   - Return value: 0 = OK, < 0 = API misuse (PHD_E_*), > 0 = a numeric fault
+ *     detected on the device (PHD_F_*), reported by phd_get_error().
+ *   - Arithmetic is IEEE-754 double.  PHD_MODE_BITWISE (default) evaluates
+ *     every expression in the reference's operation order without FMA
+ *     contraction; PHD_MODE_FAST hoists reciprocals and contracts FMAs.
+ */
+#ifndef PHD_FLUX_H
+#define PHD_FLUX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHD_ABI_VERSION 1
+
+/* API misuse */
+#define PHD_E_ARG      (-1)   /* bad argument (dim, sizes, NULL pointer) */
+#define PHD_E_HIP      (-2)   /* a HIP runtime call failed */
+#define PHD_E_MODE     (-3)   /* unsupported solver / limiter / mode */
+/* device-detected numeric faults (phd_get_error) */
+#define PHD_F_NEIGHBORS 1     /* "ERROR: incorrect neighbors!"  reconstruction.pyx:650-652 */
+#define PHD_F_MASS      2     /* "Mass less than zero in flux update"  mesh.pyx:546-548 */
+#define PHD_F_ENERGY    3     /* "Energy less than zero in flux update" mesh.pyx:549-551 */
+#define PHD_F_EXACT     4     /* "No convergence in Exact Riemann Solver" riemann.pyx:1088-1090 */
+
+/* solvers (riemann/riemann.pyx: HLL 225, HLLC 507, Exact 684) */
+#define PHD_SOLVER_HLL   0
+#define PHD_SOLVER_HLLC  1
+#define PHD_SOLVER_EXACT 2
+/* limiters (reconstruction.pyx:406-411) */
+#define PHD_LIMITER_AREPO 0
+#define PHD_LIMITER_TESS  1
+/* boundary conditions (domain/boundary.pyx: Reflective 70, Periodic 350) */
+#define PHD_BOUNDARY_REFLECTIVE 0
+#define PHD_BOUNDARY_PERIODIC   1
+/* arithmetic modes */
+#define PHD_MODE_BITWISE 0
+#define PHD_MODE_FAST    1
+/* reconstruction stages for phd_compute_states */
+#define PHD_STAGE_SPATIAL  1
+#define PHD_STAGE_TEMPORAL 2
+/* reconstruction order */
+#define PHD_ORDER_CONSTANT 0
+#define PHD_ORDER_LINEAR   1
+
+typedef struct phd_handle phd_handle;
+
+/* Cell (particle) arrays, n_cells = real + ghost.  Ghosts follow the reals. */
+typedef struct phd_cells {
+    int32_t        dim;        /* 2 or 3 */
+    int32_t        n_cells;    /* real + ghost */
+    const int8_t*  tag;        /* "tag"  */
+    const int8_t*  type;       /* "type" */
+    const int32_t* map;        /* "map": image index of each ghost (may be NULL if no ghosts) */
+    const double*  x[3];       /* "position-x".. */
+    const double*  dcom[3];    /* "dcom-x".. */
+    const double*  volume;     /* "volume" */
+    double*        prim[5];    /* primitive group */
+    double*        cons[5];    /* conservative group */
+    double*        w[3];       /* "w-x".. generator velocity */
+} phd_cells;
+
+/* Face arrays (mesh.pyx:16-46) plus the per-cell face lists (mesh.pxd:34). */
+typedef struct phd_faces {
+    int32_t        n_faces;
+    const double*  area;
+    const double*  com[3];
+    const double*  normal[3];
+    double*        velocity[3];
+    const int32_t* pair_i;
+    const int32_t* pair_j;
+    const int32_t* nbr_off;    /* n_cells + 1 offsets into nbr_face */
+    const int32_t* nbr_face;   /* face ids, ascending within each cell */
+} phd_faces;
+
+/* ---- library / handle ---------------------------------------------------- */
+int phd_abi_version(void);
+const char* phd_error_string(int code);
+/* Create a handle on `device`; allocates the handle's small device scratch
+ * (error word, reduction cells) once.  No later call allocates. */
+int phd_handle_create(phd_handle** out, int device);
+int phd_handle_destroy(phd_handle* h);
+/* Stream for all later work (a hipStream_t; NULL = default stream). */
+int phd_set_stream(phd_handle* h, void* stream);
+int phd_set_mode(phd_handle* h, int mode);
+/* Synchronising: read and clear the device error word.  *code is 0 or a
+ * PHD_F_* value, *index the first failing face (flux update, Exact solver)
+ * or cell (neighbors) in the reference's loop order. */
+int phd_get_error(phd_handle* h, int32_t* code, int64_t* index);
+
+/* ---- equation of state (equation_state/equation_state.pyx) --------------- */
+/* IdealGas.conservative_from_primitive, equation_state.pyx:43-79 */
+int phd_conservative_from_primitive(phd_handle* h, const phd_cells* c, double gamma);
+/* IdealGas.primitive_from_conservative, equation_state.pyx:81-117 */
+int phd_primitive_from_conservative(phd_handle* h, const phd_cells* c, double gamma);
+
+/* ---- time step ---------------------------------------------------------- */
+/* RiemannBase.compute_time_step, riemann.pyx:120-188.  Synchronising:
+ * writes cfl * min over real cells to *dt_out (host). */
+int phd_compute_time_step(phd_handle* h, const phd_cells* c, double gamma,
+                          double cfl, int boost, double* dt_out);
+
+/* ---- mesh motion (mesh/mesh.pyx) ---------------------------------------- */
+/* Mesh.assign_generator_velocities, mesh.pyx:363-426 (all cells) */
+int phd_assign_generator_velocities(phd_handle* h, const phd_cells* c,
+                                    double gamma, double eta, int regularize);
+/* Mesh.assign_face_velocities, mesh.pyx:428-477 (all faces) */
+int phd_assign_face_velocities(phd_handle* h, const phd_cells* c,
+                               const phd_faces* f);
+
+/* ---- reconstruction (reconstruction/reconstruction.pyx) ----------------- */
+/* PieceWiseLinear.compute_gradients, reconstruction.pyx:563-737 (real cells;
+ * the ghost transfer at 739 is phd_update_ghost_gradients). */
+int phd_compute_gradients(phd_handle* h, const phd_cells* c, const phd_faces* f,
+                          int limiter, double* grad, int32_t grad_ld);
+/* DomainManager.update_ghost_gradients (serial), domain_manager.pyx:819-837,
+ * with Reflective.update_gradients (boundary.pyx:265-307) or the periodic
+ * no-op (boundary.pyx:457-478).  lo/hi: host arrays of dim box bounds. */
+int phd_update_ghost_gradients(phd_handle* h, const phd_cells* c, int boundary,
+                               const double* lo, const double* hi,
+                               double* grad, int32_t grad_ld);
+/* PieceWiseConstant.compute_states (reconstruction.pyx:254-352) when
+ * order == PHD_ORDER_CONSTANT; PieceWiseLinear.add_spatial (741-921) and/or
+ * add_temporal (923-1072) when order == PHD_ORDER_LINEAR, selected by the
+ * `stages` bit mask.  left/right: dim+2 face arrays each (primitive group).
+ * With stages == PHD_STAGE_TEMPORAL the states are read and updated in place. */
+int phd_compute_states(phd_handle* h, const phd_cells* c, const phd_faces* f,
+                       const double* grad, int32_t grad_ld, double gamma,
+                       double dt, int boost, int gizmo, int order, int stages,
+                       double* const* left, double* const* right);
+
+/* ---- Riemann solvers (riemann/riemann.pyx) ------------------------------ */
+/* RiemannBase.compute_fluxes (74-98) with HLL (243-373), HLLC (521-682) or
+ * Exact (700-972) and RiemannBase.deboost (190-222), from materialised
+ * states.  flux: dim+2 face arrays (conservative group). */
+int phd_compute_fluxes(phd_handle* h, const phd_faces* f, int dim,
+                       double* const* left, double* const* right, double gamma,
+                       int boost, int solver, double* const* flux);
+/* Fused PieceWise{Constant,Linear}.compute_states + compute_fluxes: the face
+ * states are built in registers and never written.  Bit-identical to
+ * phd_compute_states followed by phd_compute_fluxes. */
+int phd_face_flux(phd_handle* h, const phd_cells* c, const phd_faces* f,
+                  const double* grad, int32_t grad_ld, double gamma, double dt,
+                  int boost, int gizmo, int order, int solver,
+                  double* const* flux);
+
+/* ---- conserved update (mesh/mesh.pyx) ----------------------------------- */
+/* Mesh.update_from_fluxes, mesh.pyx:479-551.  Deterministic: each real cell
+ * applies its faces in ascending face id, as the reference's face loop does,
+ * so the result is bit-identical.  Positivity faults are recorded in the
+ * error word (PHD_F_MASS / PHD_F_ENERGY at the first failing face). */
+int phd_update_from_fluxes(phd_handle* h, const phd_cells* c, const phd_faces* f,
+                           double* const* flux, double dt);
+
+/* ---- fused core step ---------------------------------------------------- */
+/* One MovingMeshMUSCLHancock hydro step on a frozen mesh with precomputed
+ * face velocities (integrate.py:396-408): gradients, ghost gradients,
+ * fused states + Riemann (at dt/2) into `flux`, conserved update at dt. */
+int phd_core_step(phd_handle* h, const phd_cells* c, const phd_faces* f,
+                  double* grad, int32_t grad_ld, int limiter, int boundary,
+                  const double* lo, const double* hi, double gamma, double dt,
+                  int boost, int gizmo, int solver, double* const* flux);
+
+/* ---- static-mesh epilogue ---------------------------------------------- */
+/* DomainManager.update_ghost_fields(particles, conservative, True)
+ * (domain_manager.pyx:759-774, Reflective.update_fields boundary.pyx:309-348)
+ * followed by IdealGas.primitive_from_conservative (equation_state.pyx:81-117)
+ * over all cells, fused (StaticMeshMUSCLHancock.evolve_timestep,
+ * integrate.py:371-377). */
+int phd_ghost_refresh_eos(phd_handle* h, const phd_cells* c, int boundary,
+                          const double* lo, const double* hi, double gamma);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHD_FLUX_H */

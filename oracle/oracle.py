@@ -1,0 +1,196 @@
+"""ctypes binding of the CPU oracle (oracle/phd_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  Arrays are host numpy arrays
+in the layout produced by phd.problems.host_arrays (reference field order).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphd_oracle.so")
+
+_lib = None
+dp = ctypes.POINTER(ctypes.c_double)
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.orc_compute_time_step.restype = ctypes.c_double
+    return _lib
+
+
+def _pp(arrs):
+    """double** over a list of float64 arrays (kept alive by the caller)."""
+    a = (ctypes.c_void_p * max(1, len(arrs)))()
+    for q, x in enumerate(arrs):
+        assert x.dtype == np.float64 and x.flags["C_CONTIGUOUS"]
+        a[q] = x.ctypes.data
+    return a
+
+
+def _p(x):
+    return ctypes.c_void_p(x.ctypes.data) if x is not None and x.size else ctypes.c_void_p(0)
+
+
+def _f64(lst):
+    return [np.ascontiguousarray(x, dtype=np.float64) for x in lst]
+
+
+def cons_from_prim(dim, gamma, prim, vol):
+    prim = _f64(prim)
+    cons = [np.zeros_like(prim[0]) for _ in range(dim + 2)]
+    lib().orc_cons_from_prim(dim, prim[0].size, ctypes.c_double(gamma), _pp(prim), _p(vol),
+                             _pp(cons))
+    return cons
+
+
+def prim_from_cons(dim, gamma, cons, vol):
+    cons = _f64(cons)
+    prim = [np.zeros_like(cons[0]) for _ in range(dim + 2)]
+    lib().orc_prim_from_cons(dim, cons[0].size, ctypes.c_double(gamma), _pp(cons), _p(vol),
+                             _pp(prim))
+    return prim
+
+
+def time_step(dim, gamma, cfl, boost, tag, prim, vol):
+    prim = _f64(prim)
+    return lib().orc_compute_time_step(dim, prim[0].size, ctypes.c_double(gamma),
+                                       ctypes.c_double(cfl), int(boost), _p(tag), _pp(prim),
+                                       _p(vol))
+
+
+def generator_velocities(dim, gamma, eta, regularize, prim, dcom, vol):
+    prim, dcom = _f64(prim), _f64(dcom)
+    w = [np.zeros_like(prim[0]) for _ in range(dim)]
+    lib().orc_generator_velocities(dim, prim[0].size, ctypes.c_double(gamma),
+                                   ctypes.c_double(eta), int(regularize), _pp(prim), _pp(dcom),
+                                   _p(vol), _pp(w))
+    return w
+
+
+def face_velocities(dim, x, w, com, pair_i, pair_j):
+    x, w, com = _f64(x), _f64(w), _f64(com)
+    fv = [np.zeros_like(com[0]) for _ in range(dim)]
+    lib().orc_face_velocities(dim, com[0].size, _pp(x), _pp(w), _pp(com), _p(pair_i),
+                              _p(pair_j), _pp(fv))
+    return fv
+
+
+def compute_gradients(a, limiter=0):
+    """Returns (grad list of dim*(dim+2) arrays, rc, bad_cell)."""
+    dim, n = a["dim"], a["n"]
+    grad = [np.zeros(n) for _ in range(dim * (dim + 2))]
+    bad = ctypes.c_int32(-1)
+    rc = lib().orc_compute_gradients(dim, n, _p(a["tag"]), _pp(a["x"]), _pp(a["dcom"]),
+                                     _p(a["volume"]), _pp(a["prim"]), _p(a["area"]),
+                                     _pp(a["com"]), _p(a["pair_i"]), _p(a["pair_j"]),
+                                     _p(a["nbr_off"]), _p(a["nbr_face"]), int(limiter),
+                                     _pp(grad), ctypes.byref(bad))
+    return grad, rc, bad.value
+
+
+def ghost_gradients(a, grad):
+    dim = a["dim"]
+    lo = np.array(a["lo"] + [0.0] * (3 - dim))
+    hi = np.array(a["hi"] + [0.0] * (3 - dim))
+    lib().orc_ghost_gradients(dim, a["n"], _p(a["tag"]), _p(a["type"]), _p(a["map"]),
+                              _pp(a["x"]), _p(lo), _p(hi), int(a["boundary"]), _pp(grad))
+    return grad
+
+
+def add_spatial(a, grad, boost=True, gizmo=True):
+    dim, nf = a["dim"], a["nf"]
+    pl = [np.zeros(nf) for _ in range(dim + 2)]
+    pr = [np.zeros(nf) for _ in range(dim + 2)]
+    lib().orc_add_spatial(dim, nf, _pp(a["prim"]), _pp(a["x"]), _pp(a["dcom"]), _pp(grad),
+                          _pp(a["com"]), _pp(a["fv"]), _p(a["pair_i"]), _p(a["pair_j"]),
+                          int(boost), int(gizmo), _pp(pl), _pp(pr))
+    return pl, pr
+
+
+def add_temporal(a, grad, dt, pl, pr, boost=True):
+    dim, nf = a["dim"], a["nf"]
+    lib().orc_add_temporal(dim, nf, ctypes.c_double(a["gamma"]), ctypes.c_double(dt),
+                           _pp(a["prim"]), _pp(grad), _pp(a["fv"]), _p(a["pair_i"]),
+                           _p(a["pair_j"]), int(boost), _pp(pl), _pp(pr))
+    return pl, pr
+
+
+def states_constant(a, boost=True):
+    dim, nf = a["dim"], a["nf"]
+    pl = [np.zeros(nf) for _ in range(dim + 2)]
+    pr = [np.zeros(nf) for _ in range(dim + 2)]
+    lib().orc_states_constant(dim, nf, _pp(a["prim"]), _pp(a["fv"]), _p(a["pair_i"]),
+                              _p(a["pair_j"]), int(boost), _pp(pl), _pp(pr))
+    return pl, pr
+
+
+def riemann(solver, dim, gamma, boost, pl, pr, normal, fv):
+    """solver: 'hll' | 'hllc' | 'exact'.  Returns (flux, rc, bad_face)."""
+    pl, pr, normal, fv = _f64(pl), _f64(pr), _f64(normal), _f64(fv)
+    nf = pl[0].size
+    flux = [np.zeros(nf) for _ in range(dim + 2)]
+    L = lib()
+    if solver == "exact":
+        bad = ctypes.c_int32(-1)
+        rc = L.orc_exact(dim, nf, ctypes.c_double(gamma), _pp(pl), _pp(pr), _pp(normal),
+                         _pp(fv), _pp(flux), ctypes.byref(bad))
+        return flux, rc, bad.value
+    fn = L.orc_hll if solver == "hll" else L.orc_hllc
+    fn(dim, nf, ctypes.c_double(gamma), int(boost), _pp(pl), _pp(pr), _pp(normal), _pp(fv),
+       _pp(flux))
+    return flux, 0, -1
+
+
+def get_waves(dl, ul, pl, dr, ur, pr, gamma):
+    sl, sc, sr = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    lib().orc_get_waves(*[ctypes.c_double(v) for v in (dl, ul, pl, dr, ur, pr, gamma)],
+                        ctypes.byref(sl), ctypes.byref(sc), ctypes.byref(sr))
+    return sl.value, sc.value, sr.value
+
+
+def update_from_fluxes(a, flux, dt, cons=None):
+    """Returns (cons after update, rc, bad_face)."""
+    dim = a["dim"]
+    cons = _f64([c.copy() for c in (a["cons"] if cons is None else cons)])
+    bad = ctypes.c_int32(-1)
+    rc = lib().orc_update_from_fluxes(dim, a["nf"], ctypes.c_double(dt), _p(a["tag"]),
+                                      _p(a["area"]), _p(a["pair_i"]), _p(a["pair_j"]),
+                                      _pp(_f64(flux)), _pp(cons), ctypes.byref(bad))
+    return cons, rc, bad.value
+
+
+def core_step(a, dt, solver=1, boost=True, gizmo=True, limiter=0):
+    """Full core step on copies; returns dict(grad, left, right, flux, cons, rc, bad)."""
+    dim, n, nf = a["dim"], a["n"], a["nf"]
+    grad = [np.zeros(n) for _ in range(dim * (dim + 2))]
+    pl = [np.zeros(nf) for _ in range(dim + 2)]
+    pr = [np.zeros(nf) for _ in range(dim + 2)]
+    flux = [np.zeros(nf) for _ in range(dim + 2)]
+    cons = _f64([c.copy() for c in a["cons"]])
+    lo = np.array(a["lo"] + [0.0] * (3 - dim))
+    hi = np.array(a["hi"] + [0.0] * (3 - dim))
+    bad = ctypes.c_int32(-1)
+    rc = lib().orc_core_step(dim, n, nf, ctypes.c_double(a["gamma"]), ctypes.c_double(dt),
+                             int(solver), int(boost), int(gizmo), int(limiter),
+                             int(a["boundary"]), _p(a["tag"]), _p(a["type"]), _p(a["map"]),
+                             _pp(a["x"]), _pp(a["dcom"]), _p(a["volume"]), _pp(a["prim"]),
+                             _pp(cons), _p(a["area"]), _pp(a["com"]), _pp(a["normal"]),
+                             _pp(a["fv"]), _p(a["pair_i"]), _p(a["pair_j"]), _p(a["nbr_off"]),
+                             _p(a["nbr_face"]), _p(lo), _p(hi), _pp(grad), _pp(pl), _pp(pr),
+                             _pp(flux), ctypes.byref(bad))
+    return {"grad": grad, "left": pl, "right": pr, "flux": flux, "cons": cons, "rc": rc,
+            "bad": bad.value}

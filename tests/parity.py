@@ -1,0 +1,40 @@
+"""Shared comparison helpers for the parity tests.
+
+Tolerance (from the parity target of the port): |got - want| <=
+1e-12 * max(|got|, |want|, s) with s = 1e-6 * max|want| over the array, a
+per-array floor for entries that are round-off-sized relative to the array's
+scale (static regions give exact zeros in one code and 1e-20-sized values in
+the other).  Bitwise agreement is reported separately.
+"""
+import numpy as np
+
+RTOL = 1e-12
+
+
+def rel_check(got, want, rtol=RTOL, floor_frac=1e-6, name=""):
+    got = np.asarray(got, dtype=np.float64)
+    want = np.asarray(want, dtype=np.float64)
+    assert got.shape == want.shape, (name, got.shape, want.shape)
+    if want.size == 0:
+        return 1.0
+    s = floor_frac * float(np.max(np.abs(want)))
+    scale = np.maximum(np.maximum(np.abs(got), np.abs(want)), s)
+    scale = np.where(scale == 0, 1.0, scale)
+    err = np.abs(got - want) / scale
+    worst = float(np.max(err))
+    assert np.all(np.isfinite(got) == np.isfinite(want)), name
+    assert worst <= rtol, "%s: worst relative difference %.3e (index %d: got %r want %r)" % (
+        name, worst, int(np.argmax(err)), got.flat[int(np.argmax(err))],
+        want.flat[int(np.argmax(err))])
+    return float(np.mean(got == want))
+
+
+def check_fields(got_list, want_list, name, mask=None, **kw):
+    fr = []
+    for q, (g, w) in enumerate(zip(got_list, want_list)):
+        g = np.asarray(g)
+        w = np.asarray(w)
+        if mask is not None:
+            g, w = g[mask], w[mask]
+        fr.append(rel_check(g, w, name="%s[%d]" % (name, q), **kw))
+    return min(fr) if fr else 1.0

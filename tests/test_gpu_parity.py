@@ -1,0 +1,168 @@
+"""GPU parity of every hot-path stage against the CPU oracle, through the C
+ABI, on the three synthetic configurations at oracle-sized meshes.
+
+Each stage is fed the same inputs on both sides (the oracle gets host copies
+of the device arrays) and compared with the tolerance of tests/parity.py;
+BITWISE kernels are expected to agree exactly except where libm pow() and
+the GPU's pow() round differently.
+"""
+import numpy as np
+import pytest
+import torch
+
+from parity import check_fields, rel_check
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("sod_2d", 24), ("kh_2d", 24), ("sedov_3d", 10)]
+
+
+def _problem(name, n):
+    from phd import problems
+    print("case %s n=%d" % (name, n), flush=True)
+    return problems.PROBLEMS[name](n=n, seed=1)
+
+
+def _np(ts):
+    return [t.detach().cpu().numpy().copy() for t in ts]
+
+
+@pytest.mark.parametrize("name,n", CASES)
+def test_stage_parity(gpu, name, n):
+    from phd import problems
+    from oracle import oracle
+
+    p = _problem(name, n)
+    pc, mesh = p.particles, p.mesh
+    dim = p.dim
+    a0 = problems.host_arrays(p)
+    real = a0["tag"] == 0
+
+    # EOS both ways (equation_state.pyx:43-117)
+    cons_ref = oracle.cons_from_prim(dim, a0["gamma"], a0["prim"], a0["volume"])
+    check_fields(_np(pc.group("conservative")), cons_ref, "cons_from_prim")
+    prim_ref = oracle.prim_from_cons(dim, a0["gamma"], a0["cons"], a0["volume"])
+    p.eos.primitive_from_conservative(pc)
+    check_fields(_np(pc.group("primitive")), prim_ref, "prim_from_cons")
+    a0 = problems.host_arrays(p)
+
+    # CFL (riemann.pyx:120-188)
+    dt = p.riemann.compute_time_step(pc, p.eos)
+    dt_ref = oracle.time_step(dim, a0["gamma"], p.riemann.cfl, p.riemann.boost, a0["tag"],
+                              a0["prim"], a0["volume"])
+    rel_check(np.array([dt]), np.array([dt_ref]), name="dt")
+
+    # generator + face velocities (mesh.pyx:363-477)
+    p.mesh.assign_generator_velocities(pc, p.eos)
+    w_ref = oracle.generator_velocities(dim, a0["gamma"], mesh.eta, mesh.regularize,
+                                        a0["prim"], a0["dcom"], a0["volume"])
+    check_fields(_np(pc.group("w")), w_ref, "w")
+    p.mesh.assign_face_velocities(pc)
+    a = problems.host_arrays(p)
+    fv_ref = oracle.face_velocities(dim, a["x"], a["w"], a["com"], a["pair_i"], a["pair_j"])
+    check_fields(a["fv"], fv_ref, "face velocity")
+
+    # gradients + limiter + ghost gradients (reconstruction.pyx:563-739)
+    recon = p.recon
+    recon.compute_gradients(pc, mesh, p.domain_manager)
+    g_ref, rc, _ = oracle.compute_gradients(a, limiter=0)
+    assert rc == 0
+    oracle.ghost_gradients(a, g_ref)
+    g_gpu = recon.grad.rows().cpu().numpy()
+    fr = check_fields([g_gpu[:, q] for q in range(len(g_ref))], g_ref, "grad")
+    print("grad bitwise fraction", fr)
+
+    # materialised states (reconstruction.pyx:741-1072) at dt/2
+    half = 0.5 * dt
+    recon.compute_states(pc, mesh, p.eos.get_gamma(), p.domain_manager, half, True)
+    L = _np(recon.left_states.group("primitive"))
+    R = _np(recon.right_states.group("primitive"))
+    pl, pr = oracle.add_spatial(a, g_ref, boost=True, gizmo=True)
+    oracle.add_temporal(a, g_ref, half, pl, pr, boost=True)
+    check_fields(L, pl, "left states")
+    check_fields(R, pr, "right states")
+
+    # Riemann from materialised states (riemann.pyx:74-98, 521-682, 190-222)
+    p.riemann.compute_fluxes(pc, mesh, recon, p.eos)
+    F_ref, _, _ = oracle.riemann("hllc", dim, a["gamma"], True, pl, pr, a["normal"], a["fv"])
+    fr = check_fields(_np(p.riemann.fluxes.group("conservative")), F_ref, "flux (states)")
+    print("flux bitwise fraction", fr)
+
+    # fused states + Riemann
+    recon.compute_states(pc, mesh, p.eos.get_gamma(), p.domain_manager, half, True)
+    p.riemann.compute_fluxes(pc, mesh, recon, p.eos)
+    check_fields(_np(p.riemann.fluxes.group("conservative")), F_ref, "flux (fused)")
+
+    # conserved update (mesh.pyx:479-551)
+    cons_ref, rc, _ = oracle.update_from_fluxes(a, F_ref, dt)
+    assert rc == 0
+    mesh.update_from_fluxes(pc, p.riemann, dt)
+    check_fields(_np(pc.group("conservative")), cons_ref, "cons update", mask=real)
+
+
+@pytest.mark.parametrize("name,n", CASES)
+def test_core_step_parity(gpu, name, n):
+    from phd import problems
+    from oracle import oracle
+
+    p = _problem(name, n)
+    step = problems.FrozenStep(p)
+    step.prologue()
+    dt = step.time_step()
+    a = problems.host_arrays(p)
+    ref = oracle.core_step(a, dt)
+    assert ref["rc"] == 0
+    step.run(dt)
+    step.check()
+    real = a["tag"] == 0
+    check_fields(_np(p.riemann.fluxes.group("conservative")), ref["flux"], "core flux")
+    check_fields(_np(p.particles.group("conservative")), ref["cons"], "core cons", mask=real)
+
+
+@pytest.mark.parametrize("solver", ["hll", "hllc", "exact"])
+@pytest.mark.parametrize("boost", [True, False])
+def test_riemann_solvers_random_states(gpu, solver, boost):
+    """Seeded random face states, all three solvers, both frames."""
+    import ctypes
+    from phd import _native as nat
+    from oracle import oracle
+
+    rng = np.random.default_rng(7)
+    nf, dim, gamma = 20000, 3, 1.4
+    dl = 10 ** rng.uniform(-1, 1, nf)
+    dr = 10 ** rng.uniform(-1, 1, nf)
+    pl_ = 10 ** rng.uniform(-1, 1, nf)
+    pr_ = 10 ** rng.uniform(-1, 1, nf)
+    vl = [rng.uniform(-0.5, 0.5, nf) for _ in range(dim)]
+    vr = [rng.uniform(-0.5, 0.5, nf) for _ in range(dim)]
+    nrm = rng.normal(size=(dim, nf))
+    nrm /= np.linalg.norm(nrm, axis=0)
+    fv = [rng.uniform(-0.5, 0.5, nf) for _ in range(dim)]
+    L = [dl] + vl + [pl_]
+    R = [dr] + vr + [pr_]
+    code = {"hll": nat.SOLVER_HLL, "hllc": nat.SOLVER_HLLC, "exact": nat.SOLVER_EXACT}[solver]
+    ref, rc, _ = oracle.riemann(solver, dim, gamma, boost, L, R, list(nrm), fv)
+    assert rc == 0
+
+    dev = torch.device("cuda", 0)
+    T = lambda xs: [torch.as_tensor(np.ascontiguousarray(x), device=dev) for x in xs]
+    Lt, Rt, Nt, Ft = T(L), T(R), T(list(nrm)), T(fv)
+    out = [torch.zeros(nf, dtype=torch.float64, device=dev) for _ in range(dim + 2)]
+    f = nat.Faces()
+    f.n_faces = nf
+    for k in range(dim):
+        f.normal[k] = Nt[k].data_ptr()
+        f.velocity[k] = Ft[k].data_ptr()
+    lib, h = nat.flux(), nat.handle()
+    nat.check(lib.phd_compute_fluxes(h, ctypes.byref(f), dim, nat.ptr_array(Lt),
+                                     nat.ptr_array(Rt), gamma, int(boost), code,
+                                     nat.ptr_array(out)), "compute_fluxes")
+    nat.raise_device_fault(h)
+    # The exact solver is built from pow() calls inside a Newton iteration
+    # (riemann.pyx:975-1090); the GPU's and glibc's pow may differ in the last
+    # place, so its absolute floor is 1e-3 of the array maximum (absolute
+    # error <= 1e-15 * max), the round-off level of a flux whose terms are of
+    # that size.  HLL/HLLC keep the default floor.
+    floor = 1e-3 if solver == "exact" else 1e-6
+    fr = check_fields(_np(out), ref, "%s boost=%s" % (solver, boost), floor_frac=floor)
+    print(solver, boost, "bitwise fraction", fr)
