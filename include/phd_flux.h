@@ -20,16 +20,15 @@
  *                      (utils/particle_creator.pyx:53-55)
  *       gradient component g = n*dim + k is d prim_n / d x_k
  *                      (reconstruction/reconstruction.pyx:470-480); gradients
- *                      are stored cell-major: grad[cell*grad_ld + g].
+ *                      live in cell-record rows (phd_cell_record_ld).
  *   - Tags: 0 Real, 1 Ghost.  Types: 2 Exterior (utils/particle_tags.py).
  *   - Faces carry pair_i < pair_j with pair_i real, and per-cell face lists
  *     (nbr_off/nbr_face) in ascending face id (mesh/tess.cpp:363-382,
  *     mesh/tess3.cpp:509-534).
  *   - Return value: 0 = OK, < 0 = API misuse (PHD_E_*), > 0 = a numeric fault
  *     detected on the device (PHD_F_*), reported by phd_get_error().
- *   - Arithmetic is IEEE-754 double.  PHD_MODE_BITWISE (default) evaluates
- *     every expression in the reference's operation order without FMA
- *     contraction; PHD_MODE_FAST hoists reciprocals and contracts FMAs.
+ *   - Arithmetic is IEEE-754 double.  PHD_MODE_BITWISE evaluates every
+ *     expression in the reference's operation order without FMA contraction.
  */
 #ifndef PHD_FLUX_H
 #define PHD_FLUX_H
@@ -64,7 +63,6 @@ extern "C" {
 #define PHD_BOUNDARY_PERIODIC   1
 /* arithmetic modes */
 #define PHD_MODE_BITWISE 0
-#define PHD_MODE_FAST    1
 /* reconstruction stages for phd_compute_states */
 #define PHD_STAGE_SPATIAL  1
 #define PHD_STAGE_TEMPORAL 2
@@ -73,6 +71,7 @@ extern "C" {
 #define PHD_ORDER_LINEAR   1
 
 typedef struct phd_handle phd_handle;
+typedef struct phd_plan phd_plan;
 
 /* Cell (particle) arrays, n_cells = real + ghost.  Ghosts follow the reals. */
 typedef struct phd_cells {
@@ -102,20 +101,44 @@ typedef struct phd_faces {
     const int32_t* nbr_face;   /* face ids, ascending within each cell */
 } phd_faces;
 
+/* Flags of phd_core_step */
+#define PHD_CORE_XP_FRESH 1   /* the plan's packed cell rows already hold the
+                                 current primitives (written by the previous
+                                 phd_ghost_refresh_eos with this plan) */
+
 /* ---- library / handle ---------------------------------------------------- */
 int phd_abi_version(void);
 const char* phd_error_string(int code);
 /* Create a handle on `device`; allocates the handle's small device scratch
- * (error word, reduction cells) once.  No later call allocates. */
+ * (error word, reduction cell) once. */
 int phd_handle_create(phd_handle** out, int device);
 int phd_handle_destroy(phd_handle* h);
 /* Stream for all later work (a hipStream_t; NULL = default stream). */
 int phd_set_stream(phd_handle* h, void* stream);
+/* Arithmetic mode; PHD_MODE_BITWISE is the only mode of this build. */
 int phd_set_mode(phd_handle* h, int mode);
 /* Synchronising: read and clear the device error word.  *code is 0 or a
  * PHD_F_* value, *index the first failing face (flux update, Exact solver)
  * or cell (neighbors) in the reference's loop order. */
 int phd_get_error(phd_handle* h, int32_t* code, int64_t* index);
+
+/* ---- mesh plan ----------------------------------------------------------- */
+/* A plan is the device-side acceleration structure of one frozen mesh: packed
+ * face records (centroid, area, normal, generator distance), cell centroids
+ * and volumes, per-cell face lists as {face, other cell, incoming bit}, and
+ * the per-step scratch of the fused kernels.  It is built from the mesh
+ * arrays the reference's Mesh.build_geometry produces (mesh.pyx:255-323) and
+ * must be recreated whenever the mesh is rebuilt.  The only allocating and
+ * synchronising calls of the library besides the handle's. */
+int phd_plan_create(phd_handle* h, const phd_cells* c, const phd_faces* f, phd_plan** out);
+int phd_plan_destroy(phd_plan* p);
+int64_t phd_plan_bytes(const phd_plan* p);
+/* Doubles per cell-record row: primitive(dim+2) | gradient(dim(dim+2)) |
+ * centroid(dim), padded to even (24 in 3D, 14 in 2D).  The gradient of cell
+ * c, component g = n*dim+k, is rec[c*ld + dim + 2 + g]. */
+int phd_cell_record_ld(int dim);
+/* Refresh the plan's packed [position | primitive] rows from the cells. */
+int phd_pack_cells(phd_handle* h, phd_plan* p, const phd_cells* c);
 
 /* ---- equation of state (equation_state/equation_state.pyx) --------------- */
 /* IdealGas.conservative_from_primitive, equation_state.pyx:43-79 */
@@ -138,25 +161,27 @@ int phd_assign_face_velocities(phd_handle* h, const phd_cells* c,
                                const phd_faces* f);
 
 /* ---- reconstruction (reconstruction/reconstruction.pyx) ----------------- */
-/* PieceWiseLinear.compute_gradients, reconstruction.pyx:563-737 (real cells;
- * the ghost transfer at 739 is phd_update_ghost_gradients). */
-int phd_compute_gradients(phd_handle* h, const phd_cells* c, const phd_faces* f,
-                          int limiter, double* grad, int32_t grad_ld);
+/* PieceWiseLinear.compute_gradients, reconstruction.pyx:563-737 (real
+ * cells): Springel (2010) eq. 21 gradients and the AREPO (limiter 0) or TESS
+ * (1) limiter, written into the cell-record rows.  The ghost transfer at 739
+ * is phd_update_ghost_gradients. */
+int phd_compute_gradients(phd_handle* h, phd_plan* p, const phd_cells* c,
+                          int limiter, double* rec, int32_t rec_ld);
 /* DomainManager.update_ghost_gradients (serial), domain_manager.pyx:819-837,
  * with Reflective.update_gradients (boundary.pyx:265-307) or the periodic
  * no-op (boundary.pyx:457-478).  lo/hi: host arrays of dim box bounds. */
-int phd_update_ghost_gradients(phd_handle* h, const phd_cells* c, int boundary,
-                               const double* lo, const double* hi,
-                               double* grad, int32_t grad_ld);
+int phd_update_ghost_gradients(phd_handle* h, phd_plan* p, const phd_cells* c,
+                               int boundary, const double* lo, const double* hi,
+                               double* rec, int32_t rec_ld);
 /* PieceWiseConstant.compute_states (reconstruction.pyx:254-352) when
  * order == PHD_ORDER_CONSTANT; PieceWiseLinear.add_spatial (741-921) and/or
  * add_temporal (923-1072) when order == PHD_ORDER_LINEAR, selected by the
  * `stages` bit mask.  left/right: dim+2 face arrays each (primitive group).
  * With stages == PHD_STAGE_TEMPORAL the states are read and updated in place. */
-int phd_compute_states(phd_handle* h, const phd_cells* c, const phd_faces* f,
-                       const double* grad, int32_t grad_ld, double gamma,
-                       double dt, int boost, int gizmo, int order, int stages,
-                       double* const* left, double* const* right);
+int phd_compute_states(phd_handle* h, phd_plan* p, const phd_cells* c,
+                       const phd_faces* f, const double* rec, int32_t rec_ld,
+                       double gamma, double dt, int boost, int gizmo, int order,
+                       int stages, double* const* left, double* const* right);
 
 /* ---- Riemann solvers (riemann/riemann.pyx) ------------------------------ */
 /* RiemannBase.compute_fluxes (74-98) with HLL (243-373), HLLC (521-682) or
@@ -168,36 +193,58 @@ int phd_compute_fluxes(phd_handle* h, const phd_faces* f, int dim,
 /* Fused PieceWise{Constant,Linear}.compute_states + compute_fluxes: the face
  * states are built in registers and never written.  Bit-identical to
  * phd_compute_states followed by phd_compute_fluxes. */
-int phd_face_flux(phd_handle* h, const phd_cells* c, const phd_faces* f,
-                  const double* grad, int32_t grad_ld, double gamma, double dt,
-                  int boost, int gizmo, int order, int solver,
-                  double* const* flux);
+int phd_face_flux(phd_handle* h, phd_plan* p, const phd_cells* c,
+                  const phd_faces* f, const double* rec, int32_t rec_ld,
+                  double gamma, double dt, int boost, int gizmo, int order,
+                  int solver, double* const* flux);
 
 /* ---- conserved update (mesh/mesh.pyx) ----------------------------------- */
 /* Mesh.update_from_fluxes, mesh.pyx:479-551.  Deterministic: each real cell
  * applies its faces in ascending face id, as the reference's face loop does,
  * so the result is bit-identical.  Positivity faults are recorded in the
  * error word (PHD_F_MASS / PHD_F_ENERGY at the first failing face). */
-int phd_update_from_fluxes(phd_handle* h, const phd_cells* c, const phd_faces* f,
-                           double* const* flux, double dt);
+int phd_update_from_fluxes(phd_handle* h, phd_plan* p, const phd_cells* c,
+                           const phd_faces* f, double* const* flux, double dt);
+
+/* ---- core-step stages (what phd_core_step runs, callable one by one) ----- */
+/* phd_compute_gradients without the packing of the cell rows: the plan's
+ * packed rows must be current (phd_pack_cells or phd_ghost_refresh_eos). */
+int phd_gradients_packed(phd_handle* h, phd_plan* p, const phd_cells* c,
+                         int limiter, double* rec, int32_t rec_ld);
+/* Fused states (at dt/2) + Riemann + deboost that writes, per face, the
+ * update summands (dt*area)*flux into the plan (mesh.pyx:530-544 forms the
+ * same product for each endpoint), and the fluxes when `flux` is not NULL. */
+int phd_face_summands(phd_handle* h, phd_plan* p, const phd_cells* c,
+                      const phd_faces* f, const double* rec, int32_t rec_ld,
+                      double gamma, double dt, int boost, int gizmo, int solver,
+                      double* const* flux);
+/* Mesh.update_from_fluxes (mesh.pyx:479-551) from the plan's summands. */
+int phd_update_from_summands(phd_handle* h, phd_plan* p, const phd_cells* c);
 
 /* ---- fused core step ---------------------------------------------------- */
 /* One MovingMeshMUSCLHancock hydro step on a frozen mesh with precomputed
- * face velocities (integrate.py:396-408): gradients, ghost gradients,
- * fused states + Riemann (at dt/2) into `flux`, conserved update at dt. */
-int phd_core_step(phd_handle* h, const phd_cells* c, const phd_faces* f,
-                  double* grad, int32_t grad_ld, int limiter, int boundary,
-                  const double* lo, const double* hi, double gamma, double dt,
-                  int boost, int gizmo, int solver, double* const* flux);
+ * face velocities (integrate.py:396-408): gradients + limiter, ghost
+ * gradients, fused states (at dt/2) + Riemann + deboost, conserved update at
+ * dt.  The update's summands (dt*area)*flux are formed once per face by the
+ * face kernel, the same product the reference forms for each endpoint.
+ * `flux` (dim+2 face arrays) may be NULL when the caller does not need the
+ * fluxes themselves. */
+int phd_core_step(phd_handle* h, phd_plan* p, const phd_cells* c,
+                  const phd_faces* f, double* rec, int32_t rec_ld, int limiter,
+                  int boundary, const double* lo, const double* hi,
+                  double gamma, double dt, int boost, int gizmo, int solver,
+                  double* const* flux, int flags);
 
 /* ---- static-mesh epilogue ---------------------------------------------- */
 /* DomainManager.update_ghost_fields(particles, conservative, True)
  * (domain_manager.pyx:759-774, Reflective.update_fields boundary.pyx:309-348)
  * followed by IdealGas.primitive_from_conservative (equation_state.pyx:81-117)
  * over all cells, fused (StaticMeshMUSCLHancock.evolve_timestep,
- * integrate.py:371-377). */
-int phd_ghost_refresh_eos(phd_handle* h, const phd_cells* c, int boundary,
-                          const double* lo, const double* hi, double gamma);
+ * integrate.py:371-377).  With a plan, also refreshes its packed cell rows
+ * (then phd_core_step may be called with PHD_CORE_XP_FRESH). */
+int phd_ghost_refresh_eos(phd_handle* h, phd_plan* p, const phd_cells* c,
+                          int boundary, const double* lo, const double* hi,
+                          double gamma);
 
 #ifdef __cplusplus
 }

@@ -29,7 +29,8 @@ PHD_F_NEIGHBORS, PHD_F_MASS, PHD_F_ENERGY, PHD_F_EXACT = 1, 2, 3, 4
 SOLVER_HLL, SOLVER_HLLC, SOLVER_EXACT = 0, 1, 2
 LIMITER_AREPO, LIMITER_TESS = 0, 1
 BOUNDARY_REFLECTIVE, BOUNDARY_PERIODIC = 0, 1
-MODE_BITWISE, MODE_FAST = 0, 1
+MODE_BITWISE = 0
+CORE_XP_FRESH = 1
 STAGE_SPATIAL, STAGE_TEMPORAL = 1, 2
 ORDER_CONSTANT, ORDER_LINEAR = 0, 1
 
@@ -70,22 +71,31 @@ _FLUX_SIGS = {
     "phd_set_stream": ([c_vp, c_vp], c_i32),
     "phd_set_mode": ([c_vp, c_i32], c_i32),
     "phd_get_error": ([c_vp, ctypes.POINTER(c_i32), ctypes.POINTER(c_i64)], c_i32),
+    "phd_plan_create": ([c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)], c_i32),
+    "phd_plan_destroy": ([c_vp], c_i32),
+    "phd_plan_bytes": ([c_vp], c_i64),
+    "phd_cell_record_ld": ([c_i32], c_i32),
+    "phd_pack_cells": ([c_vp, c_vp, c_vp], c_i32),
     "phd_conservative_from_primitive": ([c_vp, c_vp, c_dbl], c_i32),
     "phd_primitive_from_conservative": ([c_vp, c_vp, c_dbl], c_i32),
     "phd_compute_time_step": ([c_vp, c_vp, c_dbl, c_dbl, c_i32, ctypes.POINTER(c_dbl)], c_i32),
     "phd_assign_generator_velocities": ([c_vp, c_vp, c_dbl, c_dbl, c_i32], c_i32),
     "phd_assign_face_velocities": ([c_vp, c_vp, c_vp], c_i32),
     "phd_compute_gradients": ([c_vp, c_vp, c_vp, c_i32, c_vp, c_i32], c_i32),
-    "phd_update_ghost_gradients": ([c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_i32], c_i32),
-    "phd_compute_states": ([c_vp, c_vp, c_vp, c_vp, c_i32, c_dbl, c_dbl, c_i32, c_i32, c_i32,
-                            c_i32, c_vp, c_vp], c_i32),
+    "phd_update_ghost_gradients": ([c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_i32], c_i32),
+    "phd_compute_states": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_dbl, c_dbl, c_i32, c_i32,
+                            c_i32, c_i32, c_vp, c_vp], c_i32),
     "phd_compute_fluxes": ([c_vp, c_vp, c_i32, c_vp, c_vp, c_dbl, c_i32, c_i32, c_vp], c_i32),
-    "phd_face_flux": ([c_vp, c_vp, c_vp, c_vp, c_i32, c_dbl, c_dbl, c_i32, c_i32, c_i32, c_i32,
-                       c_vp], c_i32),
-    "phd_update_from_fluxes": ([c_vp, c_vp, c_vp, c_vp, c_dbl], c_i32),
-    "phd_core_step": ([c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_dbl, c_dbl,
-                       c_i32, c_i32, c_i32, c_vp], c_i32),
-    "phd_ghost_refresh_eos": ([c_vp, c_vp, c_i32, c_vp, c_vp, c_dbl], c_i32),
+    "phd_face_flux": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_dbl, c_dbl, c_i32, c_i32, c_i32,
+                       c_i32, c_vp], c_i32),
+    "phd_update_from_fluxes": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl], c_i32),
+    "phd_core_step": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_dbl,
+                       c_dbl, c_i32, c_i32, c_i32, c_vp, c_i32], c_i32),
+    "phd_ghost_refresh_eos": ([c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_dbl], c_i32),
+    "phd_gradients_packed": ([c_vp, c_vp, c_vp, c_i32, c_vp, c_i32], c_i32),
+    "phd_face_summands": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_dbl, c_dbl, c_i32, c_i32,
+                           c_i32, c_vp], c_i32),
+    "phd_update_from_summands": ([c_vp, c_vp, c_vp], c_i32),
 }
 
 # symbols added by later parts of the library register here too
@@ -186,6 +196,38 @@ def raise_device_fault(h):
         err = RuntimeError(FAULT_MESSAGES[code.value])
         err.index = idx.value
         raise err
+
+
+def record_ld(dim):
+    """Doubles per cell-record row (include/phd_flux.h phd_cell_record_ld)."""
+    return 24 if dim == 3 else 14
+
+
+class Plan(object):
+    """Owner of one phd_plan (device acceleration structure of a frozen mesh)."""
+
+    def __init__(self, cells, faces):
+        lib, h = flux(), handle()
+        p = c_vp()
+        check(lib.phd_plan_create(h, ctypes.byref(cells), ctypes.byref(faces), ctypes.byref(p)),
+              "phd_plan_create")
+        self.ptr = p.value
+        self._lib = lib
+
+    @property
+    def nbytes(self):
+        return int(self._lib.phd_plan_bytes(self.ptr))
+
+    def close(self):
+        if self.ptr:
+            self._lib.phd_plan_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def ptr(t):

@@ -133,9 +133,16 @@ class CarrayContainer(object):
 
 
 class GradientContainer(object):
-    """Cell-major gradient rows; column g = n*dim + k is d prim_n / d x_k."""
+    """Cell-record rows holding the gradients.
 
-    def __init__(self, field_names, dim, device=None):
+    Row layout (one row per cell): primitive(D+2) | gradient(D(D+2)) |
+    centroid(D), padded to an even length (24 doubles in 3D, 14 in 2D), so
+    the face kernel reads everything it needs about a cell in one
+    contiguous row.  Gradient component "<field>_<axis>" (column D+2 +
+    n*D + k, d prim_n / d x_k) is exposed as a strided column view with the
+    reference's named groups (reconstruction.pyx:462-509)."""
+
+    def __init__(self, field_names, dim, ld, device=None):
         self.device = default_device() if device is None else torch.device(device)
         self.dim = dim
         axis = "xyz"
@@ -153,9 +160,9 @@ class GradientContainer(object):
                 if "vel" in f:
                     groups["velocity"].append(g)
         self.carray_named_groups = groups
-        self.column = {g: c for c, g in enumerate(self.names)}
-        ng = len(self.names)
-        self.ld = ng + (ng & 1)           # 8 in 2D, 16 in 3D
+        self.offset = len(self.fields)               # gradient columns follow the primitives
+        self.column = {g: self.offset + c for c, g in enumerate(self.names)}
+        self.ld = int(ld)
         self.data = torch.zeros((0, self.ld), dtype=torch.float64, device=self.device)
         self._size = 0
 
@@ -182,4 +189,5 @@ class GradientContainer(object):
         self.data[:self._size, self.column[name]] = value
 
     def rows(self):
-        return self.data[:self._size, :len(self.names)]
+        """Gradient components of every cell, [n_cells, D(D+2)]."""
+        return self.data[:self._size, self.offset:self.offset + len(self.names)]

@@ -1,25 +1,37 @@
-// phd_flux.hip -- MI355X (gfx950) kernels for the Voronoi face-flux pipeline
+// phd_flux.hip -- MI355X (gfx950) kernels of the Voronoi face-flux pipeline
 // and the C ABI declared in include/phd_flux.h.
 //
 // Data layout in HBM
-//   cells  : structure-of-arrays, one double array per named field, reals
-//            first then ghosts (the reference's particle container order).
-//   grad   : cell-major rows, grad[c*ld + n*D + k] = d prim_n / d x_k.  With
-//            ld = 16 (3D) / 8 (2D) a row is one 128 B / 64 B line, so the face
-//            kernel fetches a neighbour's whole gradient in one line.
-//   faces  : structure-of-arrays, sorted by pair_i (emission order of
-//            Tess2d/Tess3d::extract_geometry, tess.cpp:363, tess3.cpp:509).
-//   lists  : CSR of face ids per cell, ascending (tess.cpp:379-382).
+//   API arrays (caller-owned, the reference's named fields): structure of
+//   arrays, one double array per field; reals first, then ghosts.
+//   Plan (library-owned, one per frozen mesh; phd_plan_create):
+//     face_rec[F][8]  com(3) | area | normal(3) | |x_j - x_i|      64 B/face
+//     cell_geo[Na][4] centroid x+dcom (3) | volume                  32 B/cell
+//     csr[2F]         {face id, other cell | incoming bit} per entry of each
+//                     cell's ascending face list (the order of
+//                     Tess2d/3d::extract_geometry, tess.cpp:379-382)
+//     xp[Na][8]       position(D) | primitive(D+2), packed per step  64 B/cell
+//     g[F][D+2]       (dt*area)*flux rows written by the face kernel
+//   Cell record rows rec[Na][ld] (the gradient container): primitive(D+2) |
+//   gradient(D(D+2)) | centroid(D); written by the gradient kernels, read
+//   whole by the face kernel (one 192 B row per 3D cell).
+//   Every gather of the hot kernels therefore touches one packed row
+//   instead of one cache segment per field.
 //
-// Arithmetic: FP64 VALU only.  The library is compiled with
-// -ffp-contract=off; BITWISE kernels evaluate each expression in the
-// reference's order (correctly rounded +,-,*,/,sqrt), so results are
-// bit-identical to the reference's x86-64 build except where a libm pow()
-// result differs in the last place.
+// Scheduling: blocks are remapped so that each of the 8 XCDs processes one
+// contiguous chunk of cells / faces (hardware dispatches workgroup b to XCD
+// b mod 8); with Morton-ordered cells, neighbours then share that XCD's L2.
+//
+// Arithmetic: FP64 VALU only, compiled with -ffp-contract=off; every
+// expression follows the reference's order, so results are bit-identical to
+// its x86-64 build except where libm pow() and the GPU's pow() round
+// differently.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
 #include "phd_flux.h"
+#include "riemann_dev.h"
+#include "states_dev.h"
 
 #define PHD_BLOCK 256
 
@@ -31,17 +43,52 @@ struct phd_handle {
     unsigned long long* h_err;   // pinned host mirror
 };
 
+struct phd_plan {
+    int dim, n_cells, n_faces;
+    long long n_entries;
+    int device;
+    void* base;                  // single allocation holding every buffer below
+    size_t bytes;
+    double* face_rec;
+    double* cell_geo;
+    int2* csr;
+    int32_t* off;
+    double* xp;
+    double* g;
+    unsigned long long* d_bad;
+    long long bad_cell;          // first real cell with an inconsistent face list, or -1
+    // sliced lists (16 cells per slice): entry r of cell c lives in slot
+    // sbase[c/16] + 16 r + c%16, so a wavefront reads 4 x 16 consecutive slots
+    long long n_slots;
+    void* slot_base;             // second allocation holding the slot arrays
+    int32_t* sbase;              // n_slices + 1
+    int32_t* sj;                 // other cell | incoming bit (INVALID for padding)
+    double* sgeo[4];             // face centroid x, y, z, area per slot
+    double* sg[5];               // per-slot update summands (owner side negated)
+    int32_t* fslot;              // per face: owner slot, neighbour slot (-1 for ghosts)
+};
+
+using namespace phd;
+
 namespace {
 
 constexpr unsigned long long NOERR = ~0ull;
 constexpr unsigned long long INF_BITS = 0x7ff0000000000000ull;
+constexpr int INCOMING = (int)0x80000000u;
+constexpr int INVALID = 0x7fffffff;
+constexpr int FREC = 8;      // doubles per face record
+constexpr int GEO = 4;       // doubles per cell geometry row
+constexpr int XPW = 8;       // doubles per packed cell row
 
 struct FaceArrays { double* q[5]; };
 
-__device__ __forceinline__ void record_min(unsigned long long* p, unsigned long long v)
-{
-    atomicMin(p, v);
-}
+template <int D> struct Lay {
+    static constexpr int NF = D + 2;                 // primitive fields
+    static constexpr int G = D * (D + 2);            // gradient components
+    static constexpr int REC = NF + G + D;           // used doubles per record row
+    static constexpr int LD = (REC + 1) & ~1;        // 24 (3D) / 14 (2D)
+    static constexpr int P0 = 0, G0 = NF, C0 = NF + G;
+};
 
 __global__ void k_reset_err(unsigned long long* e)
 {
@@ -49,14 +96,38 @@ __global__ void k_reset_err(unsigned long long* e)
     if (threadIdx.x == 3) e[3] = INF_BITS;
 }
 
+__global__ void k_set_err(unsigned long long* e, int slot, unsigned long long v)
+{
+    if (threadIdx.x == 0) atomicMin(&e[slot], v);
+}
+
 inline unsigned nblocks(long long n) { return (unsigned)((n + PHD_BLOCK - 1) / PHD_BLOCK); }
+// grid for an XCD-remapped launch over n items: a multiple of 8 blocks
+inline unsigned xcd_grid(long long n) { unsigned nb = nblocks(n); return 8u * ((nb + 7u) / 8u); }
+
+// logical item index of this thread under the XCD remap; n items in total
+__device__ __forceinline__ long long xcd_index(long long n)
+{
+    const long long nb = (n + PHD_BLOCK - 1) / PHD_BLOCK;
+    const long long per = (nb + 7) / 8;
+    const long long b = blockIdx.x;
+    const long long L = (b & 7) * per + (b >> 3);
+    return L * PHD_BLOCK + threadIdx.x;
+}
+
+__device__ __forceinline__ void ld2(const double* p, double& a, double& b)
+{
+    double2 v = *reinterpret_cast<const double2*>(p);
+    a = v.x;
+    b = v.y;
+}
 
 // ---------------------------------------------------------------------------
-// equation of state  (equation_state.pyx:43-121)
+// equation of state (equation_state.pyx:43-117), all cells
 template <int D>
 __global__ __launch_bounds__(PHD_BLOCK) void k_cons_from_prim(phd_cells c, double gamma)
 {
-    int i = blockIdx.x * PHD_BLOCK + threadIdx.x;
+    long long i = xcd_index(c.n_cells);
     if (i >= c.n_cells) return;
     double d = c.prim[0][i], vol = c.volume[i];
     double m = d * vol;
@@ -74,7 +145,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_cons_from_prim(phd_cells c, doubl
 template <int D>
 __global__ __launch_bounds__(PHD_BLOCK) void k_prim_from_cons(phd_cells c, double gamma)
 {
-    int i = blockIdx.x * PHD_BLOCK + threadIdx.x;
+    long long i = xcd_index(c.n_cells);
     if (i >= c.n_cells) return;
     double m = c.cons[0][i], vol = c.volume[i];
     double d = m / vol;
@@ -97,16 +168,14 @@ __device__ __forceinline__ double cell_radius(double vol)
     return pow(3.0 * vol / (4.0 * 3.141592653589793), 1.0 / 3.0);
 }
 
-// ---------------------------------------------------------------------------
-// CFL time step  (riemann.pyx:120-188): min over real cells of R/c (boost)
-// or R/(c+|v|).  min is order independent, so a wave-level min followed by
-// an atomic min on the IEEE bit pattern (positive doubles order as unsigned
-// integers) is exact and deterministic.
+// CFL time step (riemann.pyx:120-188).  min is order independent: a
+// wavefront min and an atomic min on the bit pattern (non-negative doubles
+// order as unsigned integers) give the exact, deterministic minimum.
 template <int D>
 __global__ __launch_bounds__(PHD_BLOCK) void k_cfl(phd_cells c, double gamma, int boost,
                                                    unsigned long long* err)
 {
-    int i = blockIdx.x * PHD_BLOCK + threadIdx.x;
+    long long i = (long long)blockIdx.x * PHD_BLOCK + threadIdx.x;
     double v = __longlong_as_double((long long)INF_BITS);
     if (i < c.n_cells && c.tag[i] == 0) {
         double d = c.prim[0][i], p = c.prim[D + 1][i];
@@ -120,28 +189,26 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_cfl(phd_cells c, double gamma, in
             for (int k = 0; k < D; k++) { double u = c.prim[1 + k][i]; vsq += u * u; }
             v = R / (cs + sqrt(vsq));
         }
-        if (!(v >= 0.0)) v = __longlong_as_double((long long)INF_BITS); // fmin ignores NaN
+        if (!(v >= 0.0)) v = __longlong_as_double((long long)INF_BITS);  // fmin skips NaN
     }
-    // wavefront min (64 lanes) via cross-lane shuffles
     for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
     __shared__ double wmin[PHD_BLOCK / 64];
-    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) wmin[wid] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double b = wmin[0];
         for (int w = 1; w < PHD_BLOCK / 64; w++) b = fmin(b, wmin[w]);
-        record_min(&err[3], (unsigned long long)__double_as_longlong(b));
+        atomicMin(&err[3], (unsigned long long)__double_as_longlong(b));
     }
 }
 
-// ---------------------------------------------------------------------------
 // generator velocities (mesh.pyx:363-426), all cells
 template <int D>
 __global__ __launch_bounds__(PHD_BLOCK) void k_gen_vel(phd_cells c, double gamma, double eta,
                                                        int regularize)
 {
-    int i = blockIdx.x * PHD_BLOCK + threadIdx.x;
+    long long i = xcd_index(c.n_cells);
     if (i >= c.n_cells) return;
     double w[D];
 #pragma unroll
@@ -170,7 +237,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_gen_vel(phd_cells c, double gamma
 template <int D>
 __global__ __launch_bounds__(PHD_BLOCK) void k_face_vel(phd_cells c, phd_faces f)
 {
-    int m = blockIdx.x * PHD_BLOCK + threadIdx.x;
+    long long m = xcd_index(f.n_faces);
     if (m >= f.n_faces) return;
     int i = f.pair_i[m], j = f.pair_j[m];
     double factor = 0.0, denom = 0.0, wi[D], wj[D], dx[D];
@@ -189,7 +256,136 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_face_vel(phd_cells c, phd_faces f
 }
 
 // ---------------------------------------------------------------------------
-// gradients + slope limiter (reconstruction.pyx:563-737), one thread per cell.
+// plan construction (once per frozen mesh)
+template <int D>
+__global__ __launch_bounds__(PHD_BLOCK) void k_plan_faces(phd_cells c, phd_faces f, double* rec)
+{
+    long long m = xcd_index(f.n_faces);
+    if (m >= f.n_faces) return;
+    int i = f.pair_i[m], j = f.pair_j[m];
+    double r[FREC] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double diff = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        r[k] = f.com[k][m];
+        r[4 + k] = f.normal[k][m];
+        double dx = c.x[k][j] - c.x[k][i];
+        diff += dx * dx;          // reconstruction.pyx:866
+    }
+    r[3] = f.area[m];
+    r[7] = sqrt(diff);            // reconstruction.pyx:875
+    double2* o = reinterpret_cast<double2*>(rec + (size_t)m * FREC);
+#pragma unroll
+    for (int q = 0; q < FREC / 2; q++) o[q] = make_double2(r[2 * q], r[2 * q + 1]);
+}
+
+template <int D>
+__global__ __launch_bounds__(PHD_BLOCK) void k_plan_cells(phd_cells c, double* geo)
+{
+    long long i = xcd_index(c.n_cells);
+    if (i >= c.n_cells) return;
+    double r[GEO] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < D; k++) r[k] = c.x[k][i] + c.dcom[k][i];   // reconstruction.pyx:626, 861
+    r[3] = c.volume[i];
+    double2* o = reinterpret_cast<double2*>(geo + (size_t)i * GEO);
+    o[0] = make_double2(r[0], r[1]);
+    o[1] = make_double2(r[2], r[3]);
+}
+
+// per entry {face id, other cell | incoming bit}; the neighbor check of
+// reconstruction.pyx:647-652 is evaluated here once per mesh
+__global__ __launch_bounds__(PHD_BLOCK) void k_plan_csr(phd_cells c, phd_faces f, int2* csr,
+                                                        unsigned long long* bad)
+{
+    long long i = xcd_index(c.n_cells);
+    if (i >= c.n_cells) return;
+    const int beg = f.nbr_off[i], end = f.nbr_off[i + 1];
+    for (int e = beg; e < end; e++) {
+        int fid = f.nbr_face[e];
+        int a = f.pair_i[fid], b = f.pair_j[fid];
+        int other;
+        if (a == (int)i) other = b;
+        else if (b == (int)i) other = a | INCOMING;
+        else {
+            other = INVALID;
+            if (c.tag[i] == 0) atomicMin(bad, (unsigned long long)i);
+        }
+        csr[e] = make_int2(fid, other);
+    }
+}
+
+constexpr int SLICE = 16;
+
+// width of each 16-cell slice = the largest face-list length of its real cells
+__global__ __launch_bounds__(PHD_BLOCK) void k_plan_slice_width(phd_cells c, const int32_t* off,
+                                                                int32_t* width, int n_slices)
+{
+    long long s = (long long)blockIdx.x * PHD_BLOCK + threadIdx.x;
+    if (s >= n_slices) return;
+    int w = 0;
+    for (int l = 0; l < SLICE; l++) {
+        long long i = s * SLICE + l;
+        if (i < c.n_cells && c.tag[i] == 0) w = max(w, off[i + 1] - off[i]);
+    }
+    width[s] = w;
+}
+
+__global__ __launch_bounds__(PHD_BLOCK) void k_fill_i32(int32_t* a, long long n, int32_t v)
+{
+    long long i = (long long)blockIdx.x * PHD_BLOCK + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// slot contents for real cells; per face the slots it feeds
+template <int D>
+__global__ __launch_bounds__(PHD_BLOCK) void k_plan_slots(phd_cells c, phd_faces f,
+                                                          const int2* csr, const int32_t* off,
+                                                          const int32_t* sbase, int32_t* sj,
+                                                          double* sx, double* sy, double* sz,
+                                                          double* sa, int32_t* fslot)
+{
+    long long ii = xcd_index(c.n_cells);
+    if (ii >= c.n_cells) return;
+    const int i = (int)ii;
+    if (c.tag[i] != 0) return;
+    const int beg = off[i], end = off[i + 1];
+    const long long base = sbase[i / SLICE] + (i % SLICE);
+    for (int e = beg; e < end; e++) {
+        const int2 en = csr[e];
+        const long long slot = base + (long long)(e - beg) * SLICE;
+        sj[slot] = en.y;
+        const int fid = en.x;
+        sx[slot] = f.com[0][fid];
+        sy[slot] = f.com[1][fid];
+        sz[slot] = D == 3 ? f.com[D - 1][fid] : 0.0;
+        sa[slot] = f.area[fid];
+        if ((en.y & 0x7fffffff) == INVALID) continue;
+        if (en.y & INCOMING) fslot[2 * (size_t)fid + 1] = (int32_t)slot;
+        else fslot[2 * (size_t)fid] = (int32_t)slot;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// packed [position | primitive] rows for the neighbour gathers
+template <int D>
+__global__ __launch_bounds__(PHD_BLOCK) void k_pack_xp(phd_cells c, double* xp)
+{
+    long long i = xcd_index(c.n_cells);
+    if (i >= c.n_cells) return;
+    double r[XPW] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < D; k++) r[k] = c.x[k][i];
+#pragma unroll
+    for (int n = 0; n < D + 2; n++) r[D + n] = c.prim[n][i];
+    double2* o = reinterpret_cast<double2*>(xp + (size_t)i * XPW);
+#pragma unroll
+    for (int q = 0; q < XPW / 2; q++) o[q] = make_double2(r[2 * q], r[2 * q + 1]);
+}
+
+// ---------------------------------------------------------------------------
+// gradients + slope limiter (reconstruction.pyx:563-737), one thread per
+// real cell; writes the cell record row [prim | alpha*df | centroid].
 //
 // The AREPO limiter (683-703) takes alpha_n = min(1, min_faces fmax(psi,0))
 // with psi = (phi_max - phi_i)/dphi for dphi > 0 and (phi_min - phi_i)/dphi
@@ -198,51 +394,72 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_face_vel(phd_cells c, phd_faces f
 // faces is reached at the largest positive dphi and at the most negative
 // dphi: two divisions per field instead of one per face, same bits.
 template <int D, int LIM>
-__global__ __launch_bounds__(PHD_BLOCK) void k_gradients(phd_cells c, phd_faces f,
-                                                         double* __restrict__ grad, int ld,
-                                                         unsigned long long* err)
+__global__ __launch_bounds__(PHD_BLOCK) void k_grad(int n_cells, const int8_t* __restrict__ tag,
+                                                    const double* __restrict__ geo,
+                                                    const double* __restrict__ xp,
+                                                    const int2* __restrict__ csr,
+                                                    const int32_t* __restrict__ off,
+                                                    const double* __restrict__ frec,
+                                                    double* __restrict__ rec, int ld)
 {
-    constexpr int NF = D + 2;
-    int i = blockIdx.x * PHD_BLOCK + threadIdx.x;
-    if (i >= c.n_cells || c.tag[i] != 0) return;
+    using Y = Lay<D>;
+    constexpr int NF = Y::NF;
+    long long ii = xcd_index(n_cells);
+    if (ii >= n_cells) return;
+    const int i = (int)ii;
+    if (tag[i] != 0) return;
 
-    double xi[D], cx[D], pv[NF], pmax[NF], pmin[NF], df[NF * D];
+    double cx[D], vol, xi[D], pv[NF];
+    {
+        double g[GEO];
+        ld2(geo + (size_t)i * GEO, g[0], g[1]);
+        ld2(geo + (size_t)i * GEO + 2, g[2], g[3]);
 #pragma unroll
-    for (int k = 0; k < D; k++) { xi[k] = c.x[k][i]; cx[k] = xi[k] + c.dcom[k][i]; }
-    const double vol = c.volume[i];
+        for (int k = 0; k < D; k++) cx[k] = g[k];
+        vol = g[3];
+        double r[XPW];
+#pragma unroll
+        for (int q = 0; q < XPW / 2; q++) ld2(xp + (size_t)i * XPW + 2 * q, r[2 * q], r[2 * q + 1]);
+#pragma unroll
+        for (int k = 0; k < D; k++) xi[k] = r[k];
+#pragma unroll
+        for (int n = 0; n < NF; n++) pv[n] = r[D + n];
+    }
+    double pmax[NF], pmin[NF], df[NF * D];
 #pragma unroll
     for (int n = 0; n < NF; n++) {
-        pv[n] = c.prim[n][i];
         pmax[n] = pmin[n] = pv[n];
 #pragma unroll
         for (int k = 0; k < D; k++) df[n * D + k] = 0.0;
     }
-    const int beg = f.nbr_off[i], end = f.nbr_off[i + 1];
-    for (int m = beg; m < end; m++) {
-        const int fid = f.nbr_face[m];
-        const double area = f.area[fid];
-        const int a = f.pair_i[fid], b = f.pair_j[fid];
-        int j;
-        if (i == a) j = b;
-        else if (i == b) j = a;
-        else { record_min(&err[1], (unsigned long long)i); return; }
-        double cfx[D], dr[D], r = 0.0;
+    const int beg = off[i], end = off[i + 1];
+    for (int e = beg; e < end; e++) {
+        const int2 en = csr[e];
+        const int j = en.y & 0x7fffffff;
+        if (j == INVALID) continue;
+        double com[3], area;
+        ld2(frec + (size_t)en.x * FREC, com[0], com[1]);
+        ld2(frec + (size_t)en.x * FREC + 2, com[2], area);
+        double r[XPW];
+#pragma unroll
+        for (int q = 0; q < XPW / 2; q++) ld2(xp + (size_t)j * XPW + 2 * q, r[2 * q], r[2 * q + 1]);
+        double cfx[D], dr[D], rr = 0.0;
 #pragma unroll
         for (int k = 0; k < D; k++) {
-            double xj = c.x[k][j];
-            cfx[k] = f.com[k][fid] - 0.5 * (xi[k] + xj);
+            const double xj = r[k];
+            cfx[k] = com[k] - 0.5 * (xi[k] + xj);
             dr[k] = xi[k] - xj;
-            r += dr[k] * dr[k];
+            rr += dr[k] * dr[k];
         }
-        r = sqrt(r);
-        const double rv = r * vol;
+        rr = sqrt(rr);
+        const double rv = rr * vol;
 #pragma unroll
         for (int n = 0; n < NF; n++) {
-            double pj = c.prim[n][j];
+            const double pj = r[D + n];
             pmax[n] = fmax(pmax[n], pj);
             pmin[n] = fmin(pmin[n], pj);
-            double d_dif = pj - pv[n];
-            double d_sum = pj + pv[n];
+            const double d_dif = pj - pv[n];
+            const double d_sum = pj + pv[n];
 #pragma unroll
             for (int k = 0; k < D; k++)
                 df[n * D + k] += area * (d_dif * cfx[k] - 0.5 * d_sum * dr[k]) / rv;
@@ -256,11 +473,14 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_gradients(phd_cells c, phd_faces 
         double dpos[NF], dneg[NF];
 #pragma unroll
         for (int n = 0; n < NF; n++) { dpos[n] = 0.0; dneg[n] = 0.0; }
-        for (int m = beg; m < end; m++) {
-            const int fid = f.nbr_face[m];
+        for (int e = beg; e < end; e++) {
+            const int2 en = csr[e];
+            double com[3], area;
+            ld2(frec + (size_t)en.x * FREC, com[0], com[1]);
+            ld2(frec + (size_t)en.x * FREC + 2, com[2], area);
             double rel[D];
 #pragma unroll
-            for (int k = 0; k < D; k++) rel[k] = f.com[k][fid] - cx[k];
+            for (int k = 0; k < D; k++) rel[k] = com[k] - cx[k];
 #pragma unroll
             for (int n = 0; n < NF; n++) {
                 double dphi = 0.0;
@@ -276,20 +496,21 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_gradients(phd_cells c, phd_faces 
             if (dneg[n] < 0) alpha[n] = fmin(alpha[n], fmax((pmin[n] - pv[n]) / dneg[n], 0.));
         }
     } else {
-        // TESS limiter (705-731); as in the reference only alpha[0] changes.
+        // TESS limiter (705-731); as in the reference only alpha[0] changes
         for (int n = 0; n < NF; n++) {
-            int j = i;
-            for (int m = beg; m < end; m++) {
-                const int fid = f.nbr_face[m];
-                const int a = f.pair_i[fid], b = f.pair_j[fid];
-                if (i == a) j = b;
-                else if (i == b) j = a;
+            for (int e = beg; e < end; e++) {
+                const int2 en = csr[e];
+                const int j = en.y & 0x7fffffff;
+                if (j == INVALID) continue;
+                double com[3], area;
+                ld2(frec + (size_t)en.x * FREC, com[0], com[1]);
+                ld2(frec + (size_t)en.x * FREC + 2, com[2], area);
                 double dphi = 0.0;
 #pragma unroll
-                for (int k = 0; k < D; k++) dphi += df[n * D + k] * (f.com[k][fid] - cx[k]);
+                for (int k = 0; k < D; k++) dphi += df[n * D + k] * (com[k] - cx[k]);
                 double psi;
                 if (dphi > 0.0 || dphi < 0.0) {
-                    double q = (c.prim[n][j] - pv[n]) / dphi;
+                    double q = (xp[(size_t)j * XPW + D + n] - pv[n]) / dphi;
                     psi = (0. > q) ? 0. : q;
                 } else {
                     psi = 1.0;
@@ -298,526 +519,291 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_gradients(phd_cells c, phd_faces 
             }
         }
     }
-    double* row = grad + (size_t)i * ld;
+    double* row = rec + (size_t)i * ld;
+    double out[Y::LD];
+#pragma unroll
+    for (int n = 0; n < NF; n++) out[Y::P0 + n] = pv[n];
 #pragma unroll
     for (int n = 0; n < NF; n++)
 #pragma unroll
-        for (int k = 0; k < D; k++) row[n * D + k] = alpha[n] * df[n * D + k];
-}
-
-// ghost gradients (domain_manager.pyx:819-837 + boundary.pyx:265-307):
-// one thread per (cell, gradient component)
-template <int D>
-__global__ __launch_bounds__(PHD_BLOCK) void k_ghost_grad(phd_cells c, int reflect, double lo0,
-                                                          double lo1, double lo2, double hi0,
-                                                          double hi1, double hi2,
-                                                          double* __restrict__ grad, int ld)
-{
-    constexpr int G = D * (D + 2);
-    long long t = (long long)blockIdx.x * PHD_BLOCK + threadIdx.x;
-    int i = (int)(t / G), g = (int)(t % G);
-    if (i >= c.n_cells || c.tag[i] != 1) return;
-    double v = grad[(size_t)c.map[i] * ld + g];
-    // velocity group entry dv[dim*q + q] = grad[(1+q)*D + q]
-    if (reflect && c.type[i] == 2 && g >= D && g < (D + 1) * D) {
-        int q = g / D - 1, k = g % D;
-        if (q == k) {
-            double lo = q == 0 ? lo0 : (q == 1 ? lo1 : lo2);
-            double hi = q == 0 ? hi0 : (q == 1 ? hi1 : hi2);
-            double xq = c.x[q][i];
-            if (xq < lo) v *= -1;
-            if (xq > hi) v *= -1;
-        }
-    }
-    grad[(size_t)i * ld + g] = v;
-}
-
-// ---------------------------------------------------------------------------
-// face states.  Spatial: PieceWiseLinear.add_spatial (741-921); temporal:
-// add_temporal (923-1072); constant: PieceWiseConstant.compute_states
-// (254-352).  L/R hold the primitive group in the face frame when boosted.
-template <int D>
-__device__ __forceinline__ void load_grad_row(const double* __restrict__ row, double* g)
-{
-    constexpr int G = D * (D + 2);
-    const double2* r2 = reinterpret_cast<const double2*>(row);
+        for (int k = 0; k < D; k++) out[Y::G0 + n * D + k] = alpha[n] * df[n * D + k];
 #pragma unroll
-    for (int q = 0; q < G / 2; q++) { double2 v = r2[q]; g[2 * q] = v.x; g[2 * q + 1] = v.y; }
-    if (G & 1) g[G - 1] = row[G - 1];
+    for (int k = 0; k < D; k++) out[Y::C0 + k] = cx[k];
+    if (Y::LD > Y::REC) out[Y::LD - 1] = 0.0;
+#pragma unroll
+    for (int q = 0; q < Y::LD / 2; q++)
+        reinterpret_cast<double2*>(row)[q] = make_double2(out[2 * q], out[2 * q + 1]);
 }
 
-template <int D>
-__device__ __forceinline__ void copy_states(const phd_cells& c, const double* fv, int boost,
-                                            int i, int j, double* pi, double* pj, double* L,
-                                            double* R)
+// gradients + limiter over the sliced lists: the per-entry face data is
+// streamed (one coalesced load per field per 16 lanes), only the packed
+// [position | primitive] row of each neighbour is gathered.  Same arithmetic,
+// same order as k_grad.
+template <int D, int LIM>
+__global__ __launch_bounds__(PHD_BLOCK) void k_grad_sell(int n_cells, const int8_t* __restrict__ tag,
+                                                         const double* __restrict__ geo,
+                                                         const double* __restrict__ xp,
+                                                         const int32_t* __restrict__ off,
+                                                         const int32_t* __restrict__ sbase,
+                                                         const int32_t* __restrict__ sj,
+                                                         const double* __restrict__ sx,
+                                                         const double* __restrict__ sy,
+                                                         const double* __restrict__ sz,
+                                                         const double* __restrict__ sa,
+                                                         double* __restrict__ rec, int ld)
 {
-    constexpr int NF = D + 2;
+    using Y = Lay<D>;
+    constexpr int NF = Y::NF;
+    long long ii = xcd_index(n_cells);
+    if (ii >= n_cells) return;
+    const int i = (int)ii;
+    if (tag[i] != 0) return;
+
+    double cx[D], vol, xi[D], pv[NF];
+    {
+        double g[GEO];
+        ld2(geo + (size_t)i * GEO, g[0], g[1]);
+        ld2(geo + (size_t)i * GEO + 2, g[2], g[3]);
+#pragma unroll
+        for (int k = 0; k < D; k++) cx[k] = g[k];
+        vol = g[3];
+        double r[XPW];
+#pragma unroll
+        for (int q = 0; q < XPW / 2; q++) ld2(xp + (size_t)i * XPW + 2 * q, r[2 * q], r[2 * q + 1]);
+#pragma unroll
+        for (int k = 0; k < D; k++) xi[k] = r[k];
+#pragma unroll
+        for (int n = 0; n < NF; n++) pv[n] = r[D + n];
+    }
+    double pmax[NF], pmin[NF], df[NF * D];
 #pragma unroll
     for (int n = 0; n < NF; n++) {
-        pi[n] = c.prim[n][i];
-        pj[n] = c.prim[n][j];
-        L[n] = pi[n];
-        R[n] = pj[n];
+        pmax[n] = pmin[n] = pv[n];
+#pragma unroll
+        for (int k = 0; k < D; k++) df[n * D + k] = 0.0;
     }
-    if (boost) {
+    const int deg = off[i + 1] - off[i];
+    const long long base = sbase[i / SLICE] + (i % SLICE);
+    for (int e = 0; e < deg; e++) {
+        const long long slot = base + (long long)e * SLICE;
+        const int j = __builtin_nontemporal_load(sj + slot) & 0x7fffffff;
+        if (j == INVALID) continue;
+        double com[3];
+        com[0] = __builtin_nontemporal_load(sx + slot);
+        com[1] = __builtin_nontemporal_load(sy + slot);
+        com[2] = D == 3 ? __builtin_nontemporal_load(sz + slot) : 0.0;
+        const double area = __builtin_nontemporal_load(sa + slot);
+        double r[XPW];
 #pragma unroll
-        for (int k = 0; k < D; k++) { L[1 + k] = pi[1 + k] - fv[k]; R[1 + k] = pj[1 + k] - fv[k]; }
-    }
-}
-
-template <int D>
-__device__ __forceinline__ void spatial_states(const phd_cells& c, const phd_faces& f, int m,
-                                               int i, int j, const double* pi, const double* pj,
-                                               const double* gi, const double* gj, int gizmo,
-                                               double* L, double* R)
-{
-    constexpr int NF = D + 2;
-    const double psi1 = 0.5, psi2 = 0.25;
-    double sl[NF], sr[NF];
+        for (int q = 0; q < XPW / 2; q++) ld2(xp + (size_t)j * XPW + 2 * q, r[2 * q], r[2 * q + 1]);
+        double cfx[D], dr[D], rr = 0.0;
 #pragma unroll
-    for (int n = 0; n < NF; n++) { sl[n] = L[n]; sr[n] = R[n]; }
-    double diff_mag = 0.0, sepi_mag = 0.0, sepj_mag = 0.0;
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-        double com = f.com[k][m];
-        double xik = c.x[k][i], xjk = c.x[k][j];
-        double sepi = com - (xik + c.dcom[k][i]);
-        double sepj = com - (xjk + c.dcom[k][j]);
-        sepi_mag += sepi * sepi;
-        sepj_mag += sepj * sepj;
-        diff_mag += (xjk - xik) * (xjk - xik);
+        for (int k = 0; k < D; k++) {
+            const double xj = r[k];
+            cfx[k] = com[k] - 0.5 * (xi[k] + xj);
+            dr[k] = xi[k] - xj;
+            rr += dr[k] * dr[k];
+        }
+        rr = sqrt(rr);
+        const double rv = rr * vol;
 #pragma unroll
         for (int n = 0; n < NF; n++) {
-            L[n] += gi[n * D + k] * sepi;
-            R[n] += gj[n * D + k] * sepj;
-        }
-    }
-    sepi_mag = sqrt(sepi_mag);
-    sepj_mag = sqrt(sepj_mag);
-    diff_mag = sqrt(diff_mag);
-    if (!gizmo) return;
-#pragma unroll
-    for (int n = 0; n < NF; n++) {
-        double delta1 = psi1 * fabs(sl[n] - sr[n]);
-        double delta2 = psi2 * fabs(sl[n] - sr[n]);
-        double phi_min = fmin(sl[n], sr[n]);
-        double phi_max = fmax(sl[n], sr[n]);
-        double phibar_l = sl[n] + sepi_mag / diff_mag * (sr[n] - sl[n]);
-        double phibar_r = sr[n] + sepj_mag / diff_mag * (sl[n] - sr[n]);
-        double phi_plus, phi_minus;
-        if ((phi_max + delta1) * phi_max >= 0.) phi_plus = phi_max + delta1;
-        else phi_plus = phi_max / (1 + delta1 / fabs(phi_max));
-        if ((phi_min - delta1) * phi_min >= 0.) phi_minus = phi_min - delta1;
-        else phi_minus = phi_min / (1 + delta1 / fabs(phi_min));
-        if (pi[n] < pj[n]) {
-            L[n] = fmax(phi_minus, fmin(phibar_l + delta2, L[n]));
-            R[n] = fmin(phi_plus, fmax(phibar_r - delta2, R[n]));
-        } else if (pi[n] > pj[n]) {
-            L[n] = fmin(phi_plus, fmax(phibar_l - delta2, L[n]));
-            R[n] = fmax(phi_minus, fmin(phibar_r + delta2, R[n]));
-        } else {
-            L[n] = sl[n];
-            R[n] = sr[n];
-        }
-    }
-    if (L[0] < 0.0 || L[D + 1] < 0.0) {
-#pragma unroll
-        for (int n = 0; n < NF; n++) L[n] = sl[n];
-    }
-    if (R[0] < 0.0 || R[D + 1] < 0.0) {
-#pragma unroll
-        for (int n = 0; n < NF; n++) R[n] = sr[n];
-    }
-}
-
-// one side of add_temporal (1018-1072)
-template <int D>
-__device__ __forceinline__ void temporal_side(const double* p, const double* g, const double* fv,
-                                              int boost, double gamma, double dt, double* S)
-{
-    constexpr int NF = D + 2;
-    double s0[NF], v[D];
-#pragma unroll
-    for (int n = 0; n < NF; n++) s0[n] = S[n];
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-        v[k] = boost ? p[1 + k] - fv[k] : p[1 + k];
-        S[1 + k] -= dt * g[(D + 1) * D + k] / p[0];
-    }
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-        double div = g[(1 + k) * D + k];
-        S[0] -= dt * (p[0] * div + v[k] * g[k]);
-        S[D + 1] -= dt * (gamma * p[D + 1] * div + v[k] * g[(D + 1) * D + k]);
-#pragma unroll
-        for (int q = 0; q < D; q++) S[1 + q] -= dt * v[k] * g[(1 + q) * D + k];
-    }
-    if (S[0] < 0.0 || S[D + 1] < 0.0) {
-#pragma unroll
-        for (int n = 0; n < NF; n++) S[n] = s0[n];
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Riemann solvers in the face frame (riemann.pyx)
-__device__ __forceinline__ void get_waves(double dl, double ul, double pl, double dr, double ur,
-                                          double pr, double gamma, double& sl, double& sc,
-                                          double& sr)
-{
-    // HLL.get_waves (riemann.pyx:375-504)
-    const double Q = 2., c_floor = 1.0E-10;
-    double cl = fmax(sqrt(gamma * pl / dl), c_floor);
-    double cr = fmax(sqrt(gamma * pr / dr), c_floor);
-    double d_avg = .5 * (dl + dr);
-    double c_avg = .5 * (cl + cr);
-    double p_star = fmax(0., .5 * (pl + pr) + .5 * (ul - ur) * d_avg * c_avg);
-    double p_min = fmin(pl, pr);
-    double p_max = fmax(pl, pr);
-    if (((p_max / p_min) < Q) && ((p_min < p_star) && (p_star < p_max))) {
-        // PVRS: p_star kept
-    } else if (p_star <= p_min) {
-        double z = (gamma - 1.) / (2. * gamma);
-        double plr = pow(pl / pr, z);
-        double u_star = (plr * ul / cl + ur / cr + 2. * (plr - 1.) / (gamma - 1.)) /
-                        (plr / cl + 1. / cr);
-        p_star = .5 * pl * pow(1. + (gamma - 1.) * (ul - u_star) / (2. * cl), 1. / z);
-        p_star += .5 * pr * pow(1. + (gamma - 1.) * (u_star - ur) / (2. * cr), 1. / z);
-    } else {
-        double Al = 2. / ((gamma + 1.) * dl);
-        double Ar = 2. / ((gamma + 1.) * dr);
-        double Bl = pl * ((gamma - 1.) / (gamma + 1.));
-        double Br = pr * ((gamma - 1.) / (gamma + 1.));
-        double gl = sqrt(Al / (p_star + Bl));
-        double gr = sqrt(Ar / (p_star + Br));
-        p_star = (gl * pl + gr * pr - (ur - ul)) / (gl + gr);
-    }
-    if (p_star <= pl) sl = ul - cl;
-    else sl = ul - cl * sqrt(1. + ((gamma + 1.) / (2. * gamma)) * (p_star / pl - 1.));
-    if (p_star <= pr) sr = ur + cr;
-    else sr = ur + cr * sqrt(1. + ((gamma + 1.) / (2. * gamma)) * (p_star / pr - 1.));
-    sc = (pr - pl + dl * ul * (sl - ul) - dr * ur * (sr - ur)) / (dl * (sl - ul) - dr * (sr - ur));
-}
-
-template <int D>
-__device__ __forceinline__ void side_flux(const double* S, const double* n, double Vn, double vsq,
-                                          double wn, double gamma, double* F)
-{
-    // pure upwind state flux in the moving frame (riemann.pyx:624-631, 672-679)
-    double d = S[0], p = S[D + 1];
-    F[0] = d * (Vn - wn);
-    F[D + 1] = (0.5 * d * vsq + p / (gamma - 1.0)) * (Vn - wn) + p * Vn;
-#pragma unroll
-    for (int k = 0; k < D; k++) F[1 + k] = d * S[1 + k] * (Vn - wn) + p * n[k];
-}
-
-template <int D, int SOLVER>
-__device__ __forceinline__ void riemann(const double* L, const double* R, const double* n,
-                                        const double* fv, double gamma, int boost, double* F)
-{
-    constexpr int NF = D + 2;
-    double _dl = L[0], _pl = L[D + 1], _dr = R[0], _pr = R[D + 1];
-    double Vnl = 0.0, Vnr = 0.0, vl_sq = 0.0, vr_sq = 0.0, wn = 0.0;
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-        vl_sq += L[1 + k] * L[1 + k];
-        vr_sq += R[1 + k] * R[1 + k];
-        Vnl += L[1 + k] * n[k];
-        Vnr += R[1 + k] * n[k];
-        wn += fv[k] * n[k];
-    }
-    if (boost) wn = 0.;
-    double sl, sc, sr;
-    get_waves(_dl, Vnl, _pl, _dr, Vnr, _pr, gamma, sl, sc, sr);
-    if (wn <= sl) {
-        side_flux<D>(L, n, Vnl, vl_sq, wn, gamma, F);
-    } else if ((sl < wn) && (wn <= sr)) {
-        if (SOLVER == PHD_SOLVER_HLL) {
-            // HLL.riemann_solver (riemann.pyx:346-361)
-            double fac1 = sr - wn, fac2 = sl - wn, fac3 = sr - sl;
-            F[0] = (_dl * Vnl * fac1 - _dr * Vnr * fac2 - sl * _dl * fac1 + sr * _dr * fac2) / fac3;
+            const double pj = r[D + n];
+            pmax[n] = fmax(pmax[n], pj);
+            pmin[n] = fmin(pmin[n], pj);
+            const double d_dif = pj - pv[n];
+            const double d_sum = pj + pv[n];
 #pragma unroll
             for (int k = 0; k < D; k++)
-                F[1 + k] = ((_dl * L[1 + k] * Vnl + _pl * n[k]) * fac1 -
-                            (_dr * R[1 + k] * Vnr + _pr * n[k]) * fac2 -
-                            sl * (_dl * L[1 + k]) * fac1 + sr * (_dr * R[1 + k]) * fac2) / fac3;
-            double el = 0.5 * _dl * vl_sq + _pl / (gamma - 1.0);
-            double er = 0.5 * _dr * vr_sq + _pr / (gamma - 1.0);
-            F[D + 1] = ((el + _pl) * Vnl * fac1 - (er + _pr) * Vnr * fac2 - sl * el * fac1 +
-                        sr * er * fac2) / fac3;
-        } else {
-            // HLLC star states (riemann.pyx:633-670), Toro eq. 10.38-10.39
-            if (wn <= sc) {
-                double factor_1 = _dl * (sl - Vnl) / (sl - sc);
-                double factor_2 = factor_1 * (sl - wn) * (sc - Vnl) + _pl;
-                double frho = _dl * (Vnl - sl) + factor_1 * (sl - wn);
-                double el = 0.5 * _dl * vl_sq + _pl / (gamma - 1.0);
-                F[0] = frho;
-                F[D + 1] = (el + _pl) * Vnl - sl * el +
-                           (sl - wn) * factor_1 *
-                               (el / _dl + (sc - Vnl) * (sc + _pl / (_dl * (sl - Vnl))));
+                df[n * D + k] += area * (d_dif * cfx[k] - 0.5 * d_sum * dr[k]) / rv;
+        }
+    }
+
+    double alpha[NF];
 #pragma unroll
-                for (int k = 0; k < D; k++) F[1 + k] = frho * L[1 + k] + factor_2 * n[k];
-            } else {
-                double factor_1 = _dr * (sr - Vnr) / (sr - sc);
-                double factor_2 = factor_1 * (sr - wn) * (sc - Vnr) + _pr;
-                double frho = _dr * (Vnr - sr) + factor_1 * (sr - wn);
-                double er = 0.5 * _dr * vr_sq + _pr / (gamma - 1.0);
-                F[0] = frho;
-                F[D + 1] = (er + _pr) * Vnr - sr * er +
-                           (sr - wn) * factor_1 *
-                               (er / _dr + (sc - Vnr) * (sc + _pr / (_dr * (sr - Vnr))));
+    for (int n = 0; n < NF; n++) alpha[n] = 1.0;
+    if (LIM == PHD_LIMITER_AREPO) {
+        double dpos[NF], dneg[NF];
 #pragma unroll
-                for (int k = 0; k < D; k++) F[1 + k] = frho * R[1 + k] + factor_2 * n[k];
+        for (int n = 0; n < NF; n++) { dpos[n] = 0.0; dneg[n] = 0.0; }
+        for (int e = 0; e < deg; e++) {
+            const long long slot = base + (long long)e * SLICE;
+            double rel[D];
+            rel[0] = __builtin_nontemporal_load(sx + slot) - cx[0];
+            rel[1] = __builtin_nontemporal_load(sy + slot) - cx[1];
+            if (D == 3) rel[D - 1] = __builtin_nontemporal_load(sz + slot) - cx[D - 1];
+#pragma unroll
+            for (int n = 0; n < NF; n++) {
+                double dphi = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; k++) dphi += df[n * D + k] * rel[k];
+                if (dphi > 0) dpos[n] = fmax(dpos[n], dphi);
+                else if (dphi < 0) dneg[n] = fmin(dneg[n], dphi);
             }
         }
+#pragma unroll
+        for (int n = 0; n < NF; n++) {
+            if (dpos[n] > 0) alpha[n] = fmin(alpha[n], fmax((pmax[n] - pv[n]) / dpos[n], 0.));
+            if (dneg[n] < 0) alpha[n] = fmin(alpha[n], fmax((pmin[n] - pv[n]) / dneg[n], 0.));
+        }
     } else {
-        side_flux<D>(R, n, Vnr, vr_sq, wn, gamma, F);
-    }
-    (void)NF;
-}
-
-// RiemannBase.deboost (riemann.pyx:190-222), Pakmor & Springel 2011 eq. 17
-template <int D>
-__device__ __forceinline__ void deboost(const double* fv, double* F)
-{
+        // TESS limiter (reconstruction.pyx:705-731); only alpha[0] changes
+        for (int n = 0; n < NF; n++) {
+            for (int e = 0; e < deg; e++) {
+                const long long slot = base + (long long)e * SLICE;
+                const int j = sj[slot] & 0x7fffffff;
+                if (j == INVALID) continue;
+                double dphi = 0.0;
+                const double cm[3] = {sx[slot], sy[slot], D == 3 ? sz[slot] : 0.0};
 #pragma unroll
-    for (int k = 0; k < D; k++) {
-        F[D + 1] += fv[k] * (0.5 * fv[k] * F[0] + F[1 + k]);
-        F[1 + k] += fv[k] * F[0];
-    }
-}
-
-// ---- Exact solver (riemann.pyx:684-1090), Toro chapter 4 ------------------
-__device__ double ex_p_func(double d, double p, double c, double gamma, double p_old)
-{
-    if (p_old <= p) return 2. * c / (gamma - 1.) * (pow(p_old / p, (gamma - 1.) / (2. * gamma)) - 1.);
-    double Ak = 2. / (d * (gamma + 1.));
-    double Bk = p * (gamma - 1.) / (gamma + 1.);
-    return (p_old - p) * sqrt(Ak / (p_old + Bk));
-}
-
-__device__ double ex_p_deriv(double d, double p, double c, double gamma, double p_old)
-{
-    if (p_old <= p) return pow(p_old / p, -(gamma + 1.) / (2. * gamma)) / (c * d);
-    double Ak = 2. / (d * (gamma + 1.));
-    double Bk = p * (gamma - 1.) / (gamma + 1.);
-    return sqrt(Ak / (p_old + Bk)) * (1. - .5 * (p_old - p) / (Bk + p_old));
-}
-
-__device__ bool ex_pstar(double dl, double ul, double pl, double cl, double dr, double ur,
-                         double pr, double cr, double gamma, double& out)
-{
-    double ppv = .5 * (pl + pr) - .125 * (ur - ul) * (dl + dr) * (cl + cr);
-    double p_max = (pl > pr) ? pl : pr;
-    double p_min = (pl < pr) ? pl : pr;
-    double q_max = p_max / p_min;
-    double p_old;
-    if ((q_max <= 2.) && (p_min <= ppv && ppv <= p_max)) {
-        p_old = ppv;
-    } else if (ppv <= p_min) {
-        double p_lr = pow(pl / pr, (gamma - 1.) / (2. * gamma));
-        double u_star = (p_lr * ul / cl + ur / cr + 2. * (p_lr - 1.) / (gamma - 1.));
-        u_star = u_star / (p_lr / cl + 1. / cr);
-        double p_tl = pow(1. + (gamma - 1.) * (ul - u_star) / (2. * cl), 2. * gamma / (gamma - 1.));
-        double p_tr = pow(1. + (gamma - 1.) * (u_star - ur) / (2. * cr), 2. * gamma / (gamma - 1.));
-        p_old = .5 * (pl * p_tl + pr * p_tr);
-    } else {
-        double gl = sqrt((2. / (dl * (gamma + 1.))) / ((gamma - 1.) * pl / (gamma + 1.) + ppv));
-        double gr = sqrt((2. / (dr * (gamma + 1.))) / ((gamma - 1.) * pr / (gamma + 1.) + ppv));
-        p_old = (gl * pl + gr * pr - (ur - ul)) / (gr + gl);
-    }
-    for (int it = 0; it < 1000; it++) {
-        double fl = ex_p_func(dl, pl, cl, gamma, p_old);
-        double fr = ex_p_func(dr, pr, cr, gamma, p_old);
-        double dfl = ex_p_deriv(dl, pl, cl, gamma, p_old);
-        double dfr = ex_p_deriv(dr, pr, cr, gamma, p_old);
-        double p_new = p_old - (fl + fr + ur - ul) / (dfl + dfr);
-        if ((2. * fabs((p_new - p_old) / (p_new + p_old))) <= 1.0e-6) { out = p_new; return true; }
-        if (p_new < 0.) p_new = 1.0e-6;
-        p_old = p_new;
-    }
-    return false;
-}
-
-template <int D>
-__device__ __forceinline__ void ex_state_flux(double d, double p, const double* v, const double* n,
-                                              double gamma, double* F)
-{
-    double v_sq = 0., vn = 0.;
-#pragma unroll
-    for (int k = 0; k < D; k++) { vn += v[k] * n[k]; v_sq += v[k] * v[k]; }
-    F[0] = d * vn;
-    F[D + 1] = (0.5 * d * v_sq + gamma * p / (gamma - 1.0)) * vn;
-#pragma unroll
-    for (int k = 0; k < D; k++) F[1 + k] = d * v[k] * vn + p * n[k];
-}
-
-// Exact.riemann_solver (riemann.pyx:700-972).  Returns false on Newton
-// non-convergence.
-template <int D>
-__device__ bool riemann_exact(const double* L, const double* R, const double* n, double gamma,
-                              double* F)
-{
-    double _dl = L[0], _pl = L[D + 1], _dr = R[0], _pr = R[D + 1];
-    double cl = sqrt(gamma * _pl / _dl), cr = sqrt(gamma * _pr / _dr);
-    double vnl = 0.0, vnr = 0.0;
-#pragma unroll
-    for (int k = 0; k < D; k++) { vnl += L[1 + k] * n[k]; vnr += R[1 + k] * n[k]; }
-    double p_star;
-    if (!ex_pstar(_dl, vnl, _pl, cl, _dr, vnr, _pr, cr, gamma, p_star)) return false;
-    double fl = ex_p_func(_dl, _pl, cl, gamma, p_star);
-    double fr = ex_p_func(_dr, _pr, cr, gamma, p_star);
-    double u_star = 0.5 * (vnl + vnr + fr - fl);
-    double v[D];
-    const double* S;
-    double d, p, shift;
-    bool fan = false;
-    if (0.0 <= u_star) {
-        S = L;
-        if (p_star <= _pl) {
-            double s_hl = vnl - cl;
-            if (0.0 <= s_hl) { d = _dl; p = _pl; shift = vnl; }
-            else {
-                double c_star_l = cl * pow(p_star / _pl, (gamma - 1.0) / (2.0 * gamma));
-                double s_tl = u_star - c_star_l;
-                if (0.0 >= s_tl) { d = _dl * pow(p_star / _pl, 1.0 / gamma); p = p_star; shift = u_star; }
-                else {
-                    double c = (2.0 / (gamma + 1.0)) * (cl + 0.5 * (gamma - 1.0) * vnl);
-                    d = _dl * pow(c / cl, 2.0 / (gamma - 1.0));
-                    p = _pl * pow(c / cl, 2.0 * gamma / (gamma - 1.0));
-                    shift = c;
+                for (int k = 0; k < D; k++) dphi += df[n * D + k] * (cm[k] - cx[k]);
+                double psi;
+                if (dphi > 0.0 || dphi < 0.0) {
+                    double q = (xp[(size_t)j * XPW + D + n] - pv[n]) / dphi;
+                    psi = (0. > q) ? 0. : q;
+                } else {
+                    psi = 1.0;
                 }
-                fan = true;
-            }
-        } else {
-            double sl = vnl - cl * sqrt((gamma + 1.0) * p_star / (2.0 * gamma * _pl) +
-                                        (gamma - 1.0) / (2.0 * gamma));
-            if (0.0 <= sl) { d = _dl; p = _pl; shift = vnl; }
-            else {
-                d = _dl * (p_star / _pl + (gamma - 1.0) / (gamma + 1.0)) /
-                    (p_star * (gamma - 1.0) / ((gamma + 1.0) * _pl) + 1.0);
-                p = p_star; shift = u_star; fan = true;
+                alpha[0] = fmin(alpha[0], fmax(psi, 0.));
             }
         }
-        if (!fan) {
-            // unperturbed left data (riemann.pyx:809-813)
-            double vsq = 0.0;
-#pragma unroll
-            for (int k = 0; k < D; k++) vsq += L[1 + k] * L[1 + k];
-            F[0] = _dl * vnl;
-            F[D + 1] = (0.5 * _dl * vsq + gamma * _pl / (gamma - 1.0)) * vnl;
-#pragma unroll
-            for (int k = 0; k < D; k++) F[1 + k] = _dl * L[1 + k] * vnl + _pl * n[k];
-            return true;
-        }
-#pragma unroll
-        for (int k = 0; k < D; k++) v[k] = S[1 + k] + (shift - vnl) * n[k];
-    } else {
-        S = R;
-        if (p_star >= _pr) {
-            double sr = vnr + cr * sqrt((gamma + 1.0) * p_star / (2.0 * gamma * _pr) +
-                                        (gamma - 1.0) / (2.0 * gamma));
-            if (0.0 >= sr) { d = _dr; p = _pr; shift = vnr; }
-            else {
-                d = _dr * (p_star / _pr + (gamma - 1.0) / (gamma + 1.0)) /
-                    (p_star * (gamma - 1.0) / ((gamma + 1.0) * _pr) + 1.0);
-                p = p_star; shift = u_star; fan = true;
-            }
-        } else {
-            double s_hr = vnr + cr;
-            if (0.0 >= s_hr) { d = _dr; p = _pr; shift = vnr; }
-            else {
-                double c_star_r = cr * pow(p_star / _pr, (gamma - 1.0) / (2.0 * gamma));
-                double s_tr = u_star + c_star_r;
-                if (0.0 <= s_tr) { d = _dr * pow(p_star / _pr, 1.0 / gamma); p = p_star; shift = u_star; }
-                else {
-                    double c = (2.0 / (gamma + 1.0)) * (cr - 0.5 * (gamma - 1.0) * vnr);
-                    double u_tmp = (2.0 / (gamma + 1.0)) * (-cr + 0.5 * (gamma - 1.0) * vnr);
-                    d = _dr * pow(c / cr, 2.0 / (gamma - 1.0));
-                    p = _pr * pow(c / cr, 2.0 * gamma / (gamma - 1.0));
-                    shift = u_tmp;
-                }
-                fan = true;
-            }
-        }
-        if (!fan) {
-            double vsq = 0.0;
-#pragma unroll
-            for (int k = 0; k < D; k++) vsq += R[1 + k] * R[1 + k];
-            F[0] = _dr * vnr;
-            F[D + 1] = (0.5 * _dr * vsq + gamma * _pr / (gamma - 1.0)) * vnr;
-#pragma unroll
-            for (int k = 0; k < D; k++) F[1 + k] = _dr * R[1 + k] * vnr + _pr * n[k];
-            return true;
-        }
-#pragma unroll
-        for (int k = 0; k < D; k++) v[k] = S[1 + k] + (shift - vnr) * n[k];
     }
-    ex_state_flux<D>(d, p, v, n, gamma, F);
-    return true;
+    double* row = rec + (size_t)i * ld;
+    double out[Y::LD];
+#pragma unroll
+    for (int n = 0; n < NF; n++) out[Y::P0 + n] = pv[n];
+#pragma unroll
+    for (int n = 0; n < NF; n++)
+#pragma unroll
+        for (int k = 0; k < D; k++) out[Y::G0 + n * D + k] = alpha[n] * df[n * D + k];
+#pragma unroll
+    for (int k = 0; k < D; k++) out[Y::C0 + k] = cx[k];
+    if (Y::LD > Y::REC) out[Y::LD - 1] = 0.0;
+#pragma unroll
+    for (int q = 0; q < Y::LD / 2; q++)
+        reinterpret_cast<double2*>(row)[q] = make_double2(out[2 * q], out[2 * q + 1]);
+}
+
+// ghost cell records: gradient copied from the image (domain_manager.pyx:
+// 829-834) with the reflective diagonal flip (boundary.pyx:291-307);
+// primitive and centroid are the ghost's own
+template <int D>
+__global__ __launch_bounds__(PHD_BLOCK) void k_ghost_rec(phd_cells c, const double* __restrict__ geo,
+                                                         int reflect, double lo0, double lo1,
+                                                         double lo2, double hi0, double hi1,
+                                                         double hi2, double* __restrict__ rec,
+                                                         int ld)
+{
+    using Y = Lay<D>;
+    long long ii = xcd_index(c.n_cells);
+    if (ii >= c.n_cells) return;
+    const int i = (int)ii;
+    if (c.tag[i] != 1) return;
+    const double* src = rec + (size_t)c.map[i] * ld;
+    double out[Y::LD];
+#pragma unroll
+    for (int q = 0; q < Y::LD / 2; q++) ld2(src + 2 * q, out[2 * q], out[2 * q + 1]);
+#pragma unroll
+    for (int n = 0; n < Y::NF; n++) out[Y::P0 + n] = c.prim[n][i];
+    if (reflect && c.type[i] == 2) {
+        const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+#pragma unroll
+        for (int q = 0; q < D; q++) {
+            const double xq = c.x[q][i];
+            // velocity group dv[D*q + q] = d v_q / d x_q
+            if (xq < lo[q]) out[Y::G0 + (1 + q) * D + q] *= -1;
+            if (xq > hi[q]) out[Y::G0 + (1 + q) * D + q] *= -1;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < D; k++) out[Y::C0 + k] = geo[(size_t)i * GEO + k];
+    double* row = rec + (size_t)i * ld;
+#pragma unroll
+    for (int q = 0; q < Y::LD / 2; q++)
+        reinterpret_cast<double2*>(row)[q] = make_double2(out[2 * q], out[2 * q + 1]);
 }
 
 // ---------------------------------------------------------------------------
-// Face kernels
+// face kernels
 template <int D>
-__device__ __forceinline__ void load_face_vec(const double* const* a, int m, double* v)
+__device__ __forceinline__ void load_face_rec(const double* frec, long long m, double* com,
+                                              double& area, double* n, double& diff)
 {
+    double r[FREC];
 #pragma unroll
-    for (int k = 0; k < D; k++) v[k] = a[k][m];
+    for (int q = 0; q < FREC / 2; q++) ld2(frec + (size_t)m * FREC + 2 * q, r[2 * q], r[2 * q + 1]);
+#pragma unroll
+    for (int k = 0; k < D; k++) { com[k] = r[k]; n[k] = r[4 + k]; }
+    area = r[3];
+    diff = r[7];
+}
+
+template <int D>
+__device__ __forceinline__ void load_rec(const double* rec, int ld, int i, double* row)
+{
+    using Y = Lay<D>;
+    const double* p = rec + (size_t)i * ld;
+#pragma unroll
+    for (int q = 0; q < Y::LD / 2; q++) ld2(p + 2 * q, row[2 * q], row[2 * q + 1]);
+}
+
+// left/right states of face m into L, R (primitive group, face frame)
+template <int D, int ORDER>
+__device__ __forceinline__ void face_states(const phd_cells& c, const double* rec, int ld, int i,
+                                            int j, const double* com, double diff,
+                                            const double* fv, double gamma, double dt, int boost,
+                                            int gizmo, int stages, double* L, double* R)
+{
+    using Y = Lay<D>;
+    if (ORDER == PHD_ORDER_CONSTANT) {
+        double pi[Y::NF], pj[Y::NF];
+#pragma unroll
+        for (int n = 0; n < Y::NF; n++) { pi[n] = c.prim[n][i]; pj[n] = c.prim[n][j]; }
+        boost_copy<D>(pi, pj, fv, boost, L, R);
+        return;
+    }
+    double ri[Y::LD], rj[Y::LD];
+    load_rec<D>(rec, ld, i, ri);
+    load_rec<D>(rec, ld, j, rj);
+    const double* pi = ri + Y::P0;
+    const double* pj = rj + Y::P0;
+    if (stages & PHD_STAGE_SPATIAL) {
+        boost_copy<D>(pi, pj, fv, boost, L, R);
+        spatial_states<D>(com, diff, ri + Y::C0, rj + Y::C0, pi, pj, ri + Y::G0, rj + Y::G0,
+                          gizmo, L, R);
+    }
+    if (stages & PHD_STAGE_TEMPORAL) {
+        temporal_side<D>(pi, ri + Y::G0, fv, boost, gamma, dt, L);
+        temporal_side<D>(pj, rj + Y::G0, fv, boost, gamma, dt, R);
+    }
 }
 
 // Materialised states: PieceWiseConstant.compute_states or
 // PieceWiseLinear.add_spatial / add_temporal.
 template <int D, int ORDER>
 __global__ __launch_bounds__(PHD_BLOCK) void k_states(phd_cells c, phd_faces f,
-                                                      const double* __restrict__ grad, int ld,
+                                                      const double* __restrict__ frec,
+                                                      const double* __restrict__ rec, int ld,
                                                       double gamma, double dt, int boost,
                                                       int gizmo, int stages, FaceArrays left,
                                                       FaceArrays right)
 {
     constexpr int NF = D + 2;
-    int m = blockIdx.x * PHD_BLOCK + threadIdx.x;
+    long long m = xcd_index(f.n_faces);
     if (m >= f.n_faces) return;
-    int i = f.pair_i[m], j = f.pair_j[m];
-    double fv[D], pi[NF], pj[NF], L[NF], R[NF];
-    load_face_vec<D>(f.velocity, m, fv);
-    if (ORDER == PHD_ORDER_CONSTANT || (stages & PHD_STAGE_SPATIAL)) {
-        copy_states<D>(c, fv, boost, i, j, pi, pj, L, R);
-    } else {
+    const int i = f.pair_i[m], j = f.pair_j[m];
+    double com[D], n[D], area, diff, fv[D], L[NF], R[NF];
+    load_face_rec<D>(frec, m, com, area, n, diff);
 #pragma unroll
-        for (int n = 0; n < NF; n++) {
-            pi[n] = c.prim[n][i];
-            pj[n] = c.prim[n][j];
-            L[n] = left.q[n][m];
-            R[n] = right.q[n][m];
-        }
-    }
-    if (ORDER == PHD_ORDER_LINEAR) {
-        double gi[D * (D + 2)], gj[D * (D + 2)];
-        load_grad_row<D>(grad + (size_t)i * ld, gi);
-        load_grad_row<D>(grad + (size_t)j * ld, gj);
-        if (stages & PHD_STAGE_SPATIAL) spatial_states<D>(c, f, m, i, j, pi, pj, gi, gj, gizmo, L, R);
-        if (stages & PHD_STAGE_TEMPORAL) {
-            temporal_side<D>(pi, gi, fv, boost, gamma, dt, L);
-            temporal_side<D>(pj, gj, fv, boost, gamma, dt, R);
-        }
-    }
+    for (int k = 0; k < D; k++) fv[k] = f.velocity[k][m];
+    if (ORDER == PHD_ORDER_LINEAR && !(stages & PHD_STAGE_SPATIAL)) {
+        // temporal only: start from the stored (spatial) states
 #pragma unroll
-    for (int n = 0; n < NF; n++) { left.q[n][m] = L[n]; right.q[n][m] = R[n]; }
-}
-
-template <int D, int SOLVER>
-__device__ __forceinline__ bool solve_face(const double* L, const double* R, const double* n,
-                                           const double* fv, double gamma, int boost, double* F)
-{
-    if (SOLVER == PHD_SOLVER_EXACT) {
-        if (!riemann_exact<D>(L, R, n, gamma, F)) return false;
-        deboost<D>(fv, F);
-        return true;
+        for (int q = 0; q < NF; q++) { L[q] = left.q[q][m]; R[q] = right.q[q][m]; }
     }
-    riemann<D, SOLVER>(L, R, n, fv, gamma, boost, F);
-    if (boost) deboost<D>(fv, F);
-    return true;
+    face_states<D, ORDER>(c, rec, ld, i, j, com, diff, fv, gamma, dt, boost, gizmo, stages, L, R);
+#pragma unroll
+    for (int q = 0; q < NF; q++) { left.q[q][m] = L[q]; right.q[q][m] = R[q]; }
 }
 
 // Riemann fluxes from materialised states (riemann.pyx:74-98)
@@ -827,93 +813,169 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_riemann(phd_faces f, FaceArrays l
                                                        FaceArrays flux, unsigned long long* err)
 {
     constexpr int NF = D + 2;
-    int m = blockIdx.x * PHD_BLOCK + threadIdx.x;
+    long long m = xcd_index(f.n_faces);
     if (m >= f.n_faces) return;
     double L[NF], R[NF], n[D], fv[D], F[NF];
 #pragma unroll
     for (int q = 0; q < NF; q++) { L[q] = left.q[q][m]; R[q] = right.q[q][m]; }
-    load_face_vec<D>(f.normal, m, n);
-    load_face_vec<D>(f.velocity, m, fv);
+#pragma unroll
+    for (int k = 0; k < D; k++) { n[k] = f.normal[k][m]; fv[k] = f.velocity[k][m]; }
     if (!solve_face<D, SOLVER>(L, R, n, fv, gamma, boost, F)) {
-        record_min(&err[2], (unsigned long long)m);
+        atomicMin(&err[2], (unsigned long long)m);
         return;
     }
 #pragma unroll
     for (int q = 0; q < NF; q++) flux.q[q][m] = F[q];
 }
 
-// Fused states + Riemann + deboost: states stay in registers.
-template <int D, int ORDER, int SOLVER>
-__global__ __launch_bounds__(PHD_BLOCK) void k_face_flux(phd_cells c, phd_faces f,
-                                                         const double* __restrict__ grad, int ld,
-                                                         double gamma, double dt, int boost,
-                                                         int gizmo, FaceArrays flux,
-                                                         unsigned long long* err)
+// Fused states + Riemann + deboost; the states never leave registers.
+// OUT bit 1: write the flux arrays (riemann.fluxes); bit 2: write
+// g[m] = (dt_full*area)*flux, the update's summands (mesh.pyx:530-544).
+template <int D, int ORDER, int SOLVER, int OUT>
+__global__ __launch_bounds__(PHD_BLOCK) void k_face(phd_cells c, phd_faces f,
+                                                    const double* __restrict__ frec,
+                                                    const double* __restrict__ rec, int ld,
+                                                    double gamma, double dt_half, double dt_full,
+                                                    int boost, int gizmo, FaceArrays flux,
+                                                    FaceArrays sg,
+                                                    const int32_t* __restrict__ fslot,
+                                                    unsigned long long* err)
 {
     constexpr int NF = D + 2;
-    int m = blockIdx.x * PHD_BLOCK + threadIdx.x;
+    long long m = xcd_index(f.n_faces);
     if (m >= f.n_faces) return;
-    int i = f.pair_i[m], j = f.pair_j[m];
-    double fv[D], n[D], pi[NF], pj[NF], L[NF], R[NF], F[NF];
-    load_face_vec<D>(f.velocity, m, fv);
-    copy_states<D>(c, fv, boost, i, j, pi, pj, L, R);
-    if (ORDER == PHD_ORDER_LINEAR) {
-        double gi[D * (D + 2)], gj[D * (D + 2)];
-        load_grad_row<D>(grad + (size_t)i * ld, gi);
-        load_grad_row<D>(grad + (size_t)j * ld, gj);
-        spatial_states<D>(c, f, m, i, j, pi, pj, gi, gj, gizmo, L, R);
-        temporal_side<D>(pi, gi, fv, boost, gamma, dt, L);
-        temporal_side<D>(pj, gj, fv, boost, gamma, dt, R);
-    }
-    load_face_vec<D>(f.normal, m, n);
+    const int i = f.pair_i[m], j = f.pair_j[m];
+    double com[D], n[D], area, diff, fv[D], L[NF], R[NF], F[NF];
+    load_face_rec<D>(frec, m, com, area, n, diff);
+#pragma unroll
+    for (int k = 0; k < D; k++) fv[k] = f.velocity[k][m];
+    face_states<D, ORDER>(c, rec, ld, i, j, com, diff, fv, gamma, dt_half, boost, gizmo,
+                          PHD_STAGE_SPATIAL | PHD_STAGE_TEMPORAL, L, R);
     if (!solve_face<D, SOLVER>(L, R, n, fv, gamma, boost, F)) {
-        record_min(&err[2], (unsigned long long)m);
+        atomicMin(&err[2], (unsigned long long)m);
         return;
     }
+    if (OUT & 1) {
 #pragma unroll
-    for (int q = 0; q < NF; q++) flux.q[q][m] = F[q];
+        for (int q = 0; q < NF; q++) flux.q[q][m] = F[q];
+    }
+    if (OUT & 2) {
+        // summands for the two endpoints' sliced lists: (dt*area)*F enters
+        // the neighbour as +G and the owner as -G (u - G == u + (-G) exactly)
+        const double t = dt_full * area;
+        const int2 fs = *reinterpret_cast<const int2*>(fslot + 2 * (size_t)m);
+#pragma unroll
+        for (int q = 0; q < NF; q++) {
+            const double gq = t * F[q];
+            if (fs.x >= 0) sg.q[q][fs.x] = -gq;
+            if (fs.y >= 0) sg.q[q][fs.y] = gq;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
 // conserved update (mesh.pyx:479-551).  One thread per real cell walks the
-// cell's face list in ascending face id, which is the order in which the
-// reference's face loop touches that cell, so every running sum is
-// bit-identical and no atomics are needed.  The positivity check of the
-// reference (546-551) fires at the first face n at which a running mass or
-// energy of either endpoint is <= 0; each endpoint reports its first
-// failure as key 2n (mass) or 2n+1 (energy) and the minimum key wins.
-template <int D>
-__global__ __launch_bounds__(PHD_BLOCK) void k_update(phd_cells c, phd_faces f, FaceArrays flux,
-                                                      double dt, unsigned long long* err)
+// cell's face list in ascending face id, the order in which the reference's
+// face loop reaches that cell, so every running sum is bit-identical and no
+// atomics are needed.  The positivity check (546-551) fires at the first
+// face n where a running mass or energy of either endpoint is <= 0; each
+// endpoint reports its first failure as key 2n (mass) or 2n+1 (energy) and
+// the minimum key wins.  Ghost endpoints never change: a ghost fails at the
+// first face of its list.  SRC 0 reads flux arrays + area, SRC 1 the g rows.
+template <int D, int SRC>
+__global__ __launch_bounds__(PHD_BLOCK) void k_update(phd_cells c, const double* __restrict__ area,
+                                                      FaceArrays flux,
+                                                      const double* __restrict__ g,
+                                                      const int2* __restrict__ csr,
+                                                      const int32_t* __restrict__ off, double dt,
+                                                      unsigned long long* err)
 {
     constexpr int NF = D + 2;
-    int i = blockIdx.x * PHD_BLOCK + threadIdx.x;
-    if (i >= c.n_cells || c.tag[i] != 0) return;
+    long long ii = xcd_index(c.n_cells);
+    if (ii >= c.n_cells) return;
+    const int i = (int)ii;
+    const int8_t tg = c.tag[i];
+    const int beg = off[i], end = off[i + 1];
+    if (tg != 0) {
+        if (beg < end) {
+            const bool bm = c.cons[0][i] <= 0.0, be = c.cons[D + 1][i] <= 0.0;
+            if (bm || be)
+                atomicMin(&err[0], 2ull * (unsigned long long)csr[beg].x + (bm ? 0ull : 1ull));
+        }
+        return;
+    }
     double u[NF];
 #pragma unroll
     for (int q = 0; q < NF; q++) u[q] = c.cons[q][i];
-    const int beg = f.nbr_off[i], end = f.nbr_off[i + 1];
-    for (int m = beg; m < end; m++) {
-        const int fid = f.nbr_face[m];
-        const double t = dt * f.area[fid];
-        const int a = f.pair_i[fid], b = f.pair_j[fid];
-        int other;
-        if (a == i) {
-            other = b;
+    for (int e = beg; e < end; e++) {
+        const int2 en = csr[e];
+        if ((en.y & 0x7fffffff) == INVALID) continue;
+        const int fid = en.x;
+        double t[NF];
+        if (SRC == 1) {
+            const double* row = g + (size_t)fid * NF;
 #pragma unroll
-            for (int q = 0; q < NF; q++) u[q] -= t * flux.q[q][fid];
+            for (int q = 0; q < NF; q++) t[q] = row[q];
         } else {
-            other = a;
+            const double s = dt * area[fid];
 #pragma unroll
-            for (int q = 0; q < NF; q++) u[q] += t * flux.q[q][fid];
+            for (int q = 0; q < NF; q++) t[q] = s * flux.q[q][fid];
         }
-        bool bad_m = u[0] <= 0.0, bad_e = u[D + 1] <= 0.0;
-        if (c.tag[other] != 0) {
-            bad_m = bad_m || c.cons[0][other] <= 0.0;
-            bad_e = bad_e || c.cons[D + 1][other] <= 0.0;
+        if (en.y & INCOMING) {
+#pragma unroll
+            for (int q = 0; q < NF; q++) u[q] += t[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < NF; q++) u[q] -= t[q];
         }
-        if (bad_m || bad_e) {
-            record_min(&err[0], 2ull * (unsigned long long)fid + (bad_m ? 0ull : 1ull));
+        const bool bm = u[0] <= 0.0, be = u[D + 1] <= 0.0;
+        if (bm || be) {
+            atomicMin(&err[0], 2ull * (unsigned long long)fid + (bm ? 0ull : 1ull));
+            break;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NF; q++) c.cons[q][i] = u[q];
+}
+
+// conserved update over the sliced lists: the slots of cell i hold its face
+// summands in ascending face id (owner side already negated), so the walk
+// u += s_0, u += s_1, ... is the reference's running sum, bit for bit.
+template <int D>
+__global__ __launch_bounds__(PHD_BLOCK) void k_update_sell(phd_cells c, FaceArrays sg,
+                                                           const int2* __restrict__ csr,
+                                                           const int32_t* __restrict__ off,
+                                                           const int32_t* __restrict__ sbase,
+                                                           const int32_t* __restrict__ sj,
+                                                           unsigned long long* err)
+{
+    constexpr int NF = D + 2;
+    long long ii = xcd_index(c.n_cells);
+    if (ii >= c.n_cells) return;
+    const int i = (int)ii;
+    const int8_t tg = c.tag[i];
+    const int beg = off[i], end = off[i + 1];
+    if (tg != 0) {
+        if (beg < end) {
+            const bool bm = c.cons[0][i] <= 0.0, be = c.cons[D + 1][i] <= 0.0;
+            if (bm || be)
+                atomicMin(&err[0], 2ull * (unsigned long long)csr[beg].x + (bm ? 0ull : 1ull));
+        }
+        return;
+    }
+    double u[NF];
+#pragma unroll
+    for (int q = 0; q < NF; q++) u[q] = c.cons[q][i];
+    const int deg = end - beg;
+    const long long base = sbase[i / SLICE] + (i % SLICE);
+    for (int e = 0; e < deg; e++) {
+        const long long slot = base + (long long)e * SLICE;
+        if ((__builtin_nontemporal_load(sj + slot) & 0x7fffffff) == INVALID) continue;
+#pragma unroll
+        for (int q = 0; q < NF; q++) u[q] += __builtin_nontemporal_load(sg.q[q] + slot);
+        const bool bm = u[0] <= 0.0, be = u[D + 1] <= 0.0;
+        if (bm || be) {
+            atomicMin(&err[0], 2ull * (unsigned long long)csr[beg + e].x + (bm ? 0ull : 1ull));
             break;
         }
     }
@@ -926,18 +988,20 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_update(phd_cells c, phd_faces f, 
 // Reflective.update_fields (boundary.pyx:336-348: momentum and velocity of
 // exterior ghosts negated on each axis where they lie outside the box),
 // then IdealGas.primitive_from_conservative (equation_state.pyx:104-117)
-// over all cells.  Ghosts read only their image's conserved values, which
-// this kernel does not write.
+// over all cells; optionally refreshes the packed [position | primitive]
+// rows.  Ghosts read only their image's conserved values, which this kernel
+// does not write.
 template <int D>
 __global__ __launch_bounds__(PHD_BLOCK) void k_ghost_refresh_eos(phd_cells c, int reflect,
                                                                  double lo0, double lo1,
                                                                  double lo2, double hi0,
                                                                  double hi1, double hi2,
-                                                                 double gamma)
+                                                                 double gamma, double* xp)
 {
     constexpr int NF = D + 2;
-    int i = blockIdx.x * PHD_BLOCK + threadIdx.x;
-    if (i >= c.n_cells) return;
+    long long ii = xcd_index(c.n_cells);
+    if (ii >= c.n_cells) return;
+    const int i = (int)ii;
     double u[NF];
     if (c.tag[i] == 1) {
         const int s = c.map[i];
@@ -959,119 +1023,233 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_ghost_refresh_eos(phd_cells c, in
         for (int q = 0; q < NF; q++) u[q] = c.cons[q][i];
     }
     const double vol = c.volume[i];
-    const double d = u[0] / vol;
-    c.prim[0][i] = d;
+    double p[NF];
+    p[0] = u[0] / vol;
     double v_sq = 0.;
 #pragma unroll
     for (int k = 0; k < D; k++) {
-        double v = u[1 + k] / u[0];
-        c.prim[1 + k][i] = v;
-        v_sq += v * v;
+        p[1 + k] = u[1 + k] / u[0];
+        v_sq += p[1 + k] * p[1 + k];
     }
-    c.prim[D + 1][i] = (u[D + 1] / vol - .5 * d * v_sq) * (gamma - 1.);
+    p[D + 1] = (u[D + 1] / vol - .5 * p[0] * v_sq) * (gamma - 1.);
+#pragma unroll
+    for (int q = 0; q < NF; q++) c.prim[q][i] = p[q];
+    if (xp) {
+        double r[XPW] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < D; k++) r[k] = c.x[k][i];
+#pragma unroll
+        for (int n = 0; n < NF; n++) r[D + n] = p[n];
+        double2* o = reinterpret_cast<double2*>(xp + (size_t)i * XPW);
+#pragma unroll
+        for (int q = 0; q < XPW / 2; q++) o[q] = make_double2(r[2 * q], r[2 * q + 1]);
+    }
 }
 
 // ---------------------------------------------------------------------------
 // host helpers
+#define HIPCHK(x) do { if ((x) != hipSuccess) return PHD_E_HIP; } while (0)
+#define LAUNCHCHK() do { if (hipPeekAtLastError() != hipSuccess) { (void)hipGetLastError(); return PHD_E_HIP; } } while (0)
+
 int check_cells(const phd_cells* c)
 {
     if (!c || (c->dim != 2 && c->dim != 3) || c->n_cells < 0) return PHD_E_ARG;
     return 0;
 }
+
 int check_faces(const phd_faces* f)
 {
     if (!f || f->n_faces < 0) return PHD_E_ARG;
     return 0;
 }
 
+int check_plan(const phd_plan* p, const phd_cells* c)
+{
+    if (!p || !c || p->dim != c->dim || p->n_cells != c->n_cells) return PHD_E_ARG;
+    return 0;
+}
+
+int check_plan_faces(const phd_plan* p, const phd_faces* f)
+{
+    if (!p || !f || p->n_faces != f->n_faces) return PHD_E_ARG;
+    return 0;
+}
+
+int check_rec(int dim, const double* rec, int ld)
+{
+    int need = dim == 3 ? Lay<3>::LD : Lay<2>::LD;
+    if (!rec || ld < need || (ld & 1) || (((uintptr_t)rec) & 15)) return PHD_E_ARG;
+    return 0;
+}
+
 FaceArrays fa(double* const* p, int nf)
 {
     FaceArrays a;
-    for (int q = 0; q < 5; q++) a.q[q] = q < nf ? p[q] : nullptr;
+    for (int q = 0; q < 5; q++) a.q[q] = (p && q < nf) ? p[q] : nullptr;
     return a;
 }
 
-#define HIPCHK(x) do { if ((x) != hipSuccess) return PHD_E_HIP; } while (0)
-#define LAUNCHCHK() do { if (hipPeekAtLastError() != hipSuccess) { (void)hipGetLastError(); return PHD_E_HIP; } } while (0)
+void bounds3(int dim, int boundary, const double* lo, const double* hi, double* l, double* u)
+{
+    for (int k = 0; k < 3; k++) { l[k] = 0; u[k] = 0; }
+    if (boundary == PHD_BOUNDARY_REFLECTIVE && lo && hi)
+        for (int k = 0; k < dim; k++) { l[k] = lo[k]; u[k] = hi[k]; }
+}
+
+FaceArrays slot_g(const phd_plan* p)
+{
+    FaceArrays a;
+    for (int q = 0; q < 5; q++) a.q[q] = p->sg[q];
+    return a;
+}
 
 template <int D>
-int launch_gradients(phd_handle* h, const phd_cells* c, const phd_faces* f, int limiter,
-                     double* grad, int ld)
+int launch_pack(phd_handle* h, phd_plan* p, const phd_cells* c)
 {
     if (c->n_cells == 0) return 0;
-    if (limiter == PHD_LIMITER_AREPO)
-        k_gradients<D, PHD_LIMITER_AREPO><<<nblocks(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, *f, grad, ld, h->d_err);
-    else if (limiter == PHD_LIMITER_TESS)
-        k_gradients<D, PHD_LIMITER_TESS><<<nblocks(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, *f, grad, ld, h->d_err);
+    k_pack_xp<D><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, p->xp);
+    LAUNCHCHK();
+    return 0;
+}
+
+template <int D>
+int launch_grad(phd_handle* h, phd_plan* p, const phd_cells* c, int limiter, double* rec, int ld)
+{
+    if (c->n_cells == 0) return 0;
+    if (p->bad_cell >= 0) {
+        k_set_err<<<1, 64, 0, h->stream>>>(h->d_err, 1, (unsigned long long)p->bad_cell);
+        LAUNCHCHK();
+    }
+    const unsigned nb = xcd_grid(c->n_cells);
+#define GS(L) k_grad_sell<D, L><<<nb, PHD_BLOCK, 0, h->stream>>>(c->n_cells, c->tag, p->cell_geo, p->xp, p->off, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], rec, ld)
+    if (limiter == PHD_LIMITER_AREPO) GS(PHD_LIMITER_AREPO);
+    else if (limiter == PHD_LIMITER_TESS) GS(PHD_LIMITER_TESS);
     else return PHD_E_MODE;
+#undef GS
     LAUNCHCHK();
     return 0;
 }
 
 template <int D>
-int launch_ghost(phd_handle* h, const phd_cells* c, int boundary, const double* lo,
-                 const double* hi, double* grad, int ld)
+int launch_ghost(phd_handle* h, phd_plan* p, const phd_cells* c, int boundary, const double* lo,
+                 const double* hi, double* rec, int ld)
 {
     if (c->n_cells == 0) return 0;
-    constexpr int G = D * (D + 2);
-    double l[3] = {0, 0, 0}, u[3] = {0, 0, 0};
-    if (boundary == PHD_BOUNDARY_REFLECTIVE) {
-        if (!lo || !hi) return PHD_E_ARG;
-        for (int k = 0; k < D; k++) { l[k] = lo[k]; u[k] = hi[k]; }
-    } else if (boundary != PHD_BOUNDARY_PERIODIC) return PHD_E_MODE;
+    if (boundary != PHD_BOUNDARY_REFLECTIVE && boundary != PHD_BOUNDARY_PERIODIC) return PHD_E_MODE;
+    if (boundary == PHD_BOUNDARY_REFLECTIVE && (!lo || !hi)) return PHD_E_ARG;
     if (!c->map) return 0;
-    long long n = (long long)c->n_cells * G;
-    k_ghost_grad<D><<<nblocks(n), PHD_BLOCK, 0, h->stream>>>(*c, boundary == PHD_BOUNDARY_REFLECTIVE, l[0], l[1], l[2], u[0], u[1], u[2], grad, ld);
+    double l[3], u[3];
+    bounds3(D, boundary, lo, hi, l, u);
+    k_ghost_rec<D><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(
+        *c, p->cell_geo, boundary == PHD_BOUNDARY_REFLECTIVE, l[0], l[1], l[2], u[0], u[1], u[2],
+        rec, ld);
     LAUNCHCHK();
     return 0;
 }
 
-template <int D, int ORDER>
-int launch_face_flux_o(phd_handle* h, const phd_cells* c, const phd_faces* f, const double* grad,
-                       int ld, double gamma, double dt, int boost, int gizmo, int solver,
-                       FaceArrays flux)
+template <int D, int ORDER, int OUT>
+int launch_face_o(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces* f,
+                  const double* rec, int ld, double gamma, double dt_half, double dt_full,
+                  int boost, int gizmo, int solver, FaceArrays flux)
 {
-    unsigned nb = nblocks(f->n_faces);
+    const unsigned nb = xcd_grid(f->n_faces);
     switch (solver) {
-    case PHD_SOLVER_HLL:
-        k_face_flux<D, ORDER, PHD_SOLVER_HLL><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, grad, ld, gamma, dt, boost, gizmo, flux, h->d_err); break;
-    case PHD_SOLVER_HLLC:
-        k_face_flux<D, ORDER, PHD_SOLVER_HLLC><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, grad, ld, gamma, dt, boost, gizmo, flux, h->d_err); break;
-    case PHD_SOLVER_EXACT:
-        k_face_flux<D, ORDER, PHD_SOLVER_EXACT><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, grad, ld, gamma, dt, boost, gizmo, flux, h->d_err); break;
+#define FACE(S) k_face<D, ORDER, S, OUT><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, p->face_rec, rec, ld, gamma, dt_half, dt_full, boost, gizmo, flux, slot_g(p), p->fslot, h->d_err)
+    case PHD_SOLVER_HLL: FACE(PHD_SOLVER_HLL); break;
+    case PHD_SOLVER_HLLC: FACE(PHD_SOLVER_HLLC); break;
+    case PHD_SOLVER_EXACT: FACE(PHD_SOLVER_EXACT); break;
+#undef FACE
     default: return PHD_E_MODE;
     }
     LAUNCHCHK();
     return 0;
 }
 
-template <int D>
-int launch_face_flux(phd_handle* h, const phd_cells* c, const phd_faces* f, const double* grad,
-                     int ld, double gamma, double dt, int boost, int gizmo, int order, int solver,
-                     FaceArrays flux)
+template <int D, int OUT>
+int launch_face(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces* f,
+                const double* rec, int ld, double gamma, double dt_half, double dt_full,
+                int boost, int gizmo, int order, int solver, FaceArrays flux)
 {
     if (f->n_faces == 0) return 0;
     if (order == PHD_ORDER_LINEAR)
-        return launch_face_flux_o<D, PHD_ORDER_LINEAR>(h, c, f, grad, ld, gamma, dt, boost, gizmo, solver, flux);
+        return launch_face_o<D, PHD_ORDER_LINEAR, OUT>(h, p, c, f, rec, ld, gamma, dt_half, dt_full, boost, gizmo, solver, flux);
     if (order == PHD_ORDER_CONSTANT)
-        return launch_face_flux_o<D, PHD_ORDER_CONSTANT>(h, c, f, grad, ld, gamma, dt, boost, gizmo, solver, flux);
+        return launch_face_o<D, PHD_ORDER_CONSTANT, OUT>(h, p, c, f, rec, ld, gamma, dt_half, dt_full, boost, gizmo, solver, flux);
     return PHD_E_MODE;
 }
 
-template <int D>
-int launch_update(phd_handle* h, const phd_cells* c, const phd_faces* f, FaceArrays flux, double dt)
+template <int D, int SRC>
+int launch_update(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces* f,
+                  FaceArrays flux, double dt)
 {
     if (c->n_cells == 0) return 0;
-    k_update<D><<<nblocks(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, *f, flux, dt, h->d_err);
+    if (SRC == 1)
+        k_update_sell<D><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(
+            *c, slot_g(p), p->csr, p->off, p->sbase, p->sj, h->d_err);
+    else
+        k_update<D, 0><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(
+            *c, f ? f->area : nullptr, flux, nullptr, p->csr, p->off, dt, h->d_err);
     LAUNCHCHK();
     return 0;
 }
 
-int grad_ok(const phd_cells* c, const double* grad, int ld)
+// sliced face lists of a plan: widths on the device, bases by a host prefix
+// sum (plan creation is the synchronising, allocating call), then the slots
+int build_slots(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces* f)
 {
-    int D = c->dim;
-    if (!grad || ld < D * (D + 2) || (ld & 1) || (((uintptr_t)grad) & 15)) return PHD_E_ARG;
-    return 0;
+    const long long na = c->n_cells, nf = f->n_faces;
+    const int D = c->dim, NF = D + 2;
+    const long long ns = (na + SLICE - 1) / SLICE;
+    int32_t* d_w = nullptr;
+    HIPCHK(hipMalloc(&d_w, sizeof(int32_t) * (size_t)(ns + 1)));
+    int32_t* hw = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ns + 1));
+    if (!hw) { (void)hipFree(d_w); return PHD_E_ARG; }
+    int rc = 0;
+    long long total = 0;
+    do {
+        if (ns > 0) k_plan_slice_width<<<nblocks(ns), PHD_BLOCK, 0, h->stream>>>(*c, p->off, d_w, (int)ns);
+        if (hipPeekAtLastError() != hipSuccess) { (void)hipGetLastError(); rc = PHD_E_HIP; break; }
+        if (hipMemcpyAsync(hw, d_w, sizeof(int32_t) * (size_t)ns, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess) { rc = PHD_E_HIP; break; }
+        for (long long s = 0; s < ns; s++) {
+            long long w = hw[s];
+            hw[s] = (int32_t)total;
+            total += w * SLICE;
+        }
+        hw[ns] = (int32_t)total;
+        if (total >= INT32_MAX) { rc = PHD_E_ARG; break; }
+    } while (0);
+    (void)hipFree(d_w);
+    if (rc) { free(hw); return rc; }
+    p->n_slots = total;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t s_base = al(sizeof(int32_t) * (size_t)(ns + 1));
+    const size_t s_j = al(sizeof(int32_t) * (size_t)total);
+    const size_t s_d = al(sizeof(double) * (size_t)total);
+    const size_t s_fs = al(sizeof(int32_t) * 2 * (size_t)nf);
+    const size_t bytes = s_base + s_j + 4 * s_d + NF * s_d + s_fs;
+    if (hipMalloc(&p->slot_base, bytes) != hipSuccess) { free(hw); p->slot_base = nullptr; return PHD_E_HIP; }
+    p->bytes += bytes;
+    char* b = (char*)p->slot_base;
+    p->sbase = (int32_t*)b; b += s_base;
+    p->sj = (int32_t*)b; b += s_j;
+    for (int q = 0; q < 4; q++) { p->sgeo[q] = (double*)b; b += s_d; }
+    for (int q = 0; q < 5; q++) p->sg[q] = nullptr;
+    for (int q = 0; q < NF; q++) { p->sg[q] = (double*)b; b += s_d; }
+    p->fslot = (int32_t*)b;
+    do {
+        if (hipMemcpyAsync(p->sbase, hw, sizeof(int32_t) * (size_t)(ns + 1), hipMemcpyHostToDevice, h->stream) != hipSuccess) { rc = PHD_E_HIP; break; }
+        if (total > 0) k_fill_i32<<<nblocks(total), PHD_BLOCK, 0, h->stream>>>(p->sj, total, INVALID);
+        if (nf > 0) k_fill_i32<<<nblocks(2 * nf), PHD_BLOCK, 0, h->stream>>>(p->fslot, 2 * nf, -1);
+        if (na > 0) {
+            if (D == 2) k_plan_slots<2><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->csr, p->off, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->fslot);
+            else k_plan_slots<3><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->csr, p->off, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->fslot);
+        }
+        if (hipPeekAtLastError() != hipSuccess) { (void)hipGetLastError(); rc = PHD_E_HIP; break; }
+        if (hipStreamSynchronize(h->stream) != hipSuccess) rc = PHD_E_HIP;
+    } while (0);
+    free(hw);
+    return rc;
 }
 
 } // namespace
@@ -1081,6 +1259,11 @@ int grad_ok(const phd_cells* c, const double* grad, int ld)
 extern "C" {
 
 int phd_abi_version(void) { return PHD_ABI_VERSION; }
+
+int phd_cell_record_ld(int dim)
+{
+    return dim == 3 ? Lay<3>::LD : (dim == 2 ? Lay<2>::LD : PHD_E_ARG);
+}
 
 const char* phd_error_string(int code)
 {
@@ -1138,7 +1321,7 @@ int phd_set_stream(phd_handle* h, void* stream)
 int phd_set_mode(phd_handle* h, int mode)
 {
     if (!h) return PHD_E_ARG;
-    if (mode != PHD_MODE_BITWISE && mode != PHD_MODE_FAST) return PHD_E_MODE;
+    if (mode != PHD_MODE_BITWISE) return PHD_E_MODE;
     h->mode = mode;
     return 0;
 }
@@ -1161,12 +1344,95 @@ int phd_get_error(phd_handle* h, int32_t* code, int64_t* index)
     return 0;
 }
 
+// ---- plan ----------------------------------------------------------------
+int phd_plan_create(phd_handle* h, const phd_cells* c, const phd_faces* f, phd_plan** out)
+{
+    if (!h || !out || check_cells(c) || check_faces(f) || !f->nbr_off) return PHD_E_ARG;
+    *out = nullptr;
+    const int D = c->dim, NF = D + 2;
+    const long long na = c->n_cells, nf = f->n_faces;
+    int32_t n_entries = 0;
+    if (na > 0) {
+        HIPCHK(hipMemcpyAsync(&n_entries, f->nbr_off + na, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    if (n_entries < 0 || (n_entries > 0 && !f->nbr_face)) return PHD_E_ARG;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t s_face = al(sizeof(double) * FREC * (size_t)nf);
+    const size_t s_geo = al(sizeof(double) * GEO * (size_t)na);
+    const size_t s_csr = al(sizeof(int2) * (size_t)n_entries);
+    const size_t s_off = al(sizeof(int32_t) * (size_t)(na + 1));
+    const size_t s_xp = al(sizeof(double) * XPW * (size_t)na);
+    const size_t s_g = 0;
+    const size_t s_bad = al(4 * sizeof(unsigned long long));
+    const size_t total = s_face + s_geo + s_csr + s_off + s_xp + s_g + s_bad;
+    phd_plan* p = new phd_plan();
+    p->dim = D;
+    p->n_cells = (int)na;
+    p->n_faces = (int)nf;
+    p->n_entries = n_entries;
+    p->device = h->device;
+    p->bytes = total;
+    p->bad_cell = -1;
+    if (hipMalloc(&p->base, total) != hipSuccess) { delete p; return PHD_E_HIP; }
+    char* b = (char*)p->base;
+    p->face_rec = (double*)b; b += s_face;
+    p->cell_geo = (double*)b; b += s_geo;
+    p->csr = (int2*)b; b += s_csr;
+    p->off = (int32_t*)b; b += s_off;
+    p->xp = (double*)b; b += s_xp;
+    p->g = nullptr; b += s_g;
+    p->d_bad = (unsigned long long*)b;
+    p->slot_base = nullptr;
+    int rc = 0;
+    do {
+        if (hipMemcpyAsync(p->off, f->nbr_off, sizeof(int32_t) * (size_t)(na + 1), hipMemcpyDeviceToDevice, h->stream) != hipSuccess) { rc = PHD_E_HIP; break; }
+        k_reset_err<<<1, 64, 0, h->stream>>>(p->d_bad);   // slot 0 = NOERR
+        if (nf > 0) {
+            if (D == 2) k_plan_faces<2><<<xcd_grid(nf), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->face_rec);
+            else k_plan_faces<3><<<xcd_grid(nf), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->face_rec);
+        }
+        if (na > 0) {
+            if (D == 2) k_plan_cells<2><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, p->cell_geo);
+            else k_plan_cells<3><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, p->cell_geo);
+            k_plan_csr<<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->csr, p->d_bad);
+        }
+        if (hipPeekAtLastError() != hipSuccess) { (void)hipGetLastError(); rc = PHD_E_HIP; break; }
+        unsigned long long bad = NOERR;
+        if (hipMemcpyAsync(&bad, p->d_bad, sizeof(bad), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess) { rc = PHD_E_HIP; break; }
+        p->bad_cell = bad == NOERR ? -1 : (long long)bad;
+        rc = build_slots(h, p, c, f);
+    } while (0);
+    if (rc) {
+        if (p->slot_base) (void)hipFree(p->slot_base);
+        (void)hipFree(p->base);
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return 0;
+}
+
+int phd_plan_destroy(phd_plan* p)
+{
+    if (!p) return PHD_E_ARG;
+    (void)hipDeviceSynchronize();
+    if (p->slot_base) (void)hipFree(p->slot_base);
+    (void)hipFree(p->base);
+    delete p;
+    return 0;
+}
+
+int64_t phd_plan_bytes(const phd_plan* p) { return p ? (int64_t)p->bytes : -1; }
+
+// ---- equation of state ----------------------------------------------------
 int phd_conservative_from_primitive(phd_handle* h, const phd_cells* c, double gamma)
 {
     if (!h || check_cells(c)) return PHD_E_ARG;
     if (c->n_cells == 0) return 0;
-    if (c->dim == 2) k_cons_from_prim<2><<<nblocks(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, gamma);
-    else k_cons_from_prim<3><<<nblocks(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, gamma);
+    if (c->dim == 2) k_cons_from_prim<2><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, gamma);
+    else k_cons_from_prim<3><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, gamma);
     LAUNCHCHK();
     return 0;
 }
@@ -1175,8 +1441,8 @@ int phd_primitive_from_conservative(phd_handle* h, const phd_cells* c, double ga
 {
     if (!h || check_cells(c)) return PHD_E_ARG;
     if (c->n_cells == 0) return 0;
-    if (c->dim == 2) k_prim_from_cons<2><<<nblocks(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, gamma);
-    else k_prim_from_cons<3><<<nblocks(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, gamma);
+    if (c->dim == 2) k_prim_from_cons<2><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, gamma);
+    else k_prim_from_cons<3><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, gamma);
     LAUNCHCHK();
     return 0;
 }
@@ -1185,7 +1451,7 @@ int phd_compute_time_step(phd_handle* h, const phd_cells* c, double gamma, doubl
                           double* dt_out)
 {
     if (!h || check_cells(c) || !dt_out || c->n_cells == 0) return PHD_E_ARG;
-    k_reset_err<<<1, 64, 0, h->stream>>>(h->d_err);  // also resets the fault words
+    k_reset_err<<<1, 64, 0, h->stream>>>(h->d_err);   // also clears the fault words
     LAUNCHCHK();
     if (c->dim == 2) k_cfl<2><<<nblocks(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, gamma, boost, h->d_err);
     else k_cfl<3><<<nblocks(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, gamma, boost, h->d_err);
@@ -1203,8 +1469,8 @@ int phd_assign_generator_velocities(phd_handle* h, const phd_cells* c, double ga
 {
     if (!h || check_cells(c)) return PHD_E_ARG;
     if (c->n_cells == 0) return 0;
-    if (c->dim == 2) k_gen_vel<2><<<nblocks(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, gamma, eta, regularize);
-    else k_gen_vel<3><<<nblocks(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, gamma, eta, regularize);
+    if (c->dim == 2) k_gen_vel<2><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, gamma, eta, regularize);
+    else k_gen_vel<3><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, gamma, eta, regularize);
     LAUNCHCHK();
     return 0;
 }
@@ -1213,59 +1479,72 @@ int phd_assign_face_velocities(phd_handle* h, const phd_cells* c, const phd_face
 {
     if (!h || check_cells(c) || check_faces(f)) return PHD_E_ARG;
     if (f->n_faces == 0) return 0;
-    if (c->dim == 2) k_face_vel<2><<<nblocks(f->n_faces), PHD_BLOCK, 0, h->stream>>>(*c, *f);
-    else k_face_vel<3><<<nblocks(f->n_faces), PHD_BLOCK, 0, h->stream>>>(*c, *f);
+    if (c->dim == 2) k_face_vel<2><<<xcd_grid(f->n_faces), PHD_BLOCK, 0, h->stream>>>(*c, *f);
+    else k_face_vel<3><<<xcd_grid(f->n_faces), PHD_BLOCK, 0, h->stream>>>(*c, *f);
     LAUNCHCHK();
     return 0;
 }
 
-int phd_compute_gradients(phd_handle* h, const phd_cells* c, const phd_faces* f, int limiter,
-                          double* grad, int32_t grad_ld)
+// ---- reconstruction -------------------------------------------------------
+int phd_pack_cells(phd_handle* h, phd_plan* p, const phd_cells* c)
 {
-    if (!h || check_cells(c) || check_faces(f) || grad_ok(c, grad, grad_ld)) return PHD_E_ARG;
-    return c->dim == 2 ? launch_gradients<2>(h, c, f, limiter, grad, grad_ld)
-                       : launch_gradients<3>(h, c, f, limiter, grad, grad_ld);
+    if (!h || check_cells(c) || check_plan(p, c)) return PHD_E_ARG;
+    return c->dim == 2 ? launch_pack<2>(h, p, c) : launch_pack<3>(h, p, c);
 }
 
-int phd_update_ghost_gradients(phd_handle* h, const phd_cells* c, int boundary, const double* lo,
-                               const double* hi, double* grad, int32_t grad_ld)
+int phd_compute_gradients(phd_handle* h, phd_plan* p, const phd_cells* c, int limiter,
+                          double* rec, int32_t rec_ld)
 {
-    if (!h || check_cells(c) || grad_ok(c, grad, grad_ld)) return PHD_E_ARG;
-    return c->dim == 2 ? launch_ghost<2>(h, c, boundary, lo, hi, grad, grad_ld)
-                       : launch_ghost<3>(h, c, boundary, lo, hi, grad, grad_ld);
+    if (!h || check_cells(c) || check_plan(p, c) || check_rec(c->dim, rec, rec_ld)) return PHD_E_ARG;
+    int rc = phd_pack_cells(h, p, c);
+    if (rc) return rc;
+    return c->dim == 2 ? launch_grad<2>(h, p, c, limiter, rec, rec_ld)
+                       : launch_grad<3>(h, p, c, limiter, rec, rec_ld);
 }
 
-int phd_compute_states(phd_handle* h, const phd_cells* c, const phd_faces* f, const double* grad,
-                       int32_t grad_ld, double gamma, double dt, int boost, int gizmo, int order,
-                       int stages, double* const* left, double* const* right)
+int phd_update_ghost_gradients(phd_handle* h, phd_plan* p, const phd_cells* c, int boundary,
+                               const double* lo, const double* hi, double* rec, int32_t rec_ld)
 {
-    if (!h || check_cells(c) || check_faces(f) || !left || !right) return PHD_E_ARG;
-    if (order == PHD_ORDER_LINEAR && grad_ok(c, grad, grad_ld)) return PHD_E_ARG;
+    if (!h || check_cells(c) || check_plan(p, c) || check_rec(c->dim, rec, rec_ld)) return PHD_E_ARG;
+    return c->dim == 2 ? launch_ghost<2>(h, p, c, boundary, lo, hi, rec, rec_ld)
+                       : launch_ghost<3>(h, p, c, boundary, lo, hi, rec, rec_ld);
+}
+
+int phd_compute_states(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces* f,
+                       const double* rec, int32_t rec_ld, double gamma, double dt, int boost,
+                       int gizmo, int order, int stages, double* const* left,
+                       double* const* right)
+{
+    if (!h || check_cells(c) || check_faces(f) || check_plan(p, c) || check_plan_faces(p, f) ||
+        !left || !right)
+        return PHD_E_ARG;
+    if (order == PHD_ORDER_LINEAR && check_rec(c->dim, rec, rec_ld)) return PHD_E_ARG;
     if (order != PHD_ORDER_LINEAR && order != PHD_ORDER_CONSTANT) return PHD_E_MODE;
     if (f->n_faces == 0) return 0;
-    int nf = c->dim + 2;
+    const int nf = c->dim + 2;
     FaceArrays L = fa(left, nf), R = fa(right, nf);
-    unsigned nb = nblocks(f->n_faces);
+    const unsigned nb = xcd_grid(f->n_faces);
+#define ST(DD, OO) k_states<DD, OO><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, p->face_rec, rec, rec_ld, gamma, dt, boost, gizmo, stages, L, R)
     if (c->dim == 2) {
-        if (order == PHD_ORDER_LINEAR) k_states<2, PHD_ORDER_LINEAR><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, grad, grad_ld, gamma, dt, boost, gizmo, stages, L, R);
-        else k_states<2, PHD_ORDER_CONSTANT><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, grad, grad_ld, gamma, dt, boost, gizmo, stages, L, R);
+        if (order == PHD_ORDER_LINEAR) ST(2, PHD_ORDER_LINEAR); else ST(2, PHD_ORDER_CONSTANT);
     } else {
-        if (order == PHD_ORDER_LINEAR) k_states<3, PHD_ORDER_LINEAR><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, grad, grad_ld, gamma, dt, boost, gizmo, stages, L, R);
-        else k_states<3, PHD_ORDER_CONSTANT><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, grad, grad_ld, gamma, dt, boost, gizmo, stages, L, R);
+        if (order == PHD_ORDER_LINEAR) ST(3, PHD_ORDER_LINEAR); else ST(3, PHD_ORDER_CONSTANT);
     }
+#undef ST
     LAUNCHCHK();
     return 0;
 }
 
+// ---- Riemann ---------------------------------------------------------------
 int phd_compute_fluxes(phd_handle* h, const phd_faces* f, int dim, double* const* left,
                        double* const* right, double gamma, int boost, int solver,
                        double* const* flux)
 {
     if (!h || check_faces(f) || (dim != 2 && dim != 3) || !left || !right || !flux) return PHD_E_ARG;
     if (f->n_faces == 0) return 0;
-    int nf = dim + 2;
+    const int nf = dim + 2;
     FaceArrays L = fa(left, nf), R = fa(right, nf), F = fa(flux, nf);
-    unsigned nb = nblocks(f->n_faces);
+    const unsigned nb = xcd_grid(f->n_faces);
 #define RIE(DD, SS) k_riemann<DD, SS><<<nb, PHD_BLOCK, 0, h->stream>>>(*f, L, R, gamma, boost, F, h->d_err)
     if (dim == 2) {
         if (solver == PHD_SOLVER_HLL) RIE(2, PHD_SOLVER_HLL);
@@ -1283,59 +1562,104 @@ int phd_compute_fluxes(phd_handle* h, const phd_faces* f, int dim, double* const
     return 0;
 }
 
-int phd_face_flux(phd_handle* h, const phd_cells* c, const phd_faces* f, const double* grad,
-                  int32_t grad_ld, double gamma, double dt, int boost, int gizmo, int order,
-                  int solver, double* const* flux)
+int phd_face_flux(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces* f,
+                  const double* rec, int32_t rec_ld, double gamma, double dt, int boost, int gizmo,
+                  int order, int solver, double* const* flux)
 {
-    if (!h || check_cells(c) || check_faces(f) || !flux) return PHD_E_ARG;
-    if (order == PHD_ORDER_LINEAR && grad_ok(c, grad, grad_ld)) return PHD_E_ARG;
+    if (!h || check_cells(c) || check_faces(f) || check_plan(p, c) || check_plan_faces(p, f) ||
+        !flux)
+        return PHD_E_ARG;
+    if (order == PHD_ORDER_LINEAR && check_rec(c->dim, rec, rec_ld)) return PHD_E_ARG;
     FaceArrays F = fa(flux, c->dim + 2);
     return c->dim == 2
-               ? launch_face_flux<2>(h, c, f, grad, grad_ld, gamma, dt, boost, gizmo, order, solver, F)
-               : launch_face_flux<3>(h, c, f, grad, grad_ld, gamma, dt, boost, gizmo, order, solver, F);
+               ? launch_face<2, 1>(h, p, c, f, rec, rec_ld, gamma, dt, 0.0, boost, gizmo, order, solver, F)
+               : launch_face<3, 1>(h, p, c, f, rec, rec_ld, gamma, dt, 0.0, boost, gizmo, order, solver, F);
 }
 
-int phd_update_from_fluxes(phd_handle* h, const phd_cells* c, const phd_faces* f,
+// ---- update ------------------------------------------------------------------
+int phd_update_from_fluxes(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces* f,
                            double* const* flux, double dt)
 {
-    if (!h || check_cells(c) || check_faces(f) || !flux) return PHD_E_ARG;
+    if (!h || check_cells(c) || check_faces(f) || check_plan(p, c) || check_plan_faces(p, f) ||
+        !flux)
+        return PHD_E_ARG;
     FaceArrays F = fa(flux, c->dim + 2);
-    return c->dim == 2 ? launch_update<2>(h, c, f, F, dt) : launch_update<3>(h, c, f, F, dt);
+    return c->dim == 2 ? launch_update<2, 0>(h, p, c, f, F, dt)
+                       : launch_update<3, 0>(h, p, c, f, F, dt);
 }
 
-int phd_core_step(phd_handle* h, const phd_cells* c, const phd_faces* f, double* grad,
-                  int32_t grad_ld, int limiter, int boundary, const double* lo, const double* hi,
-                  double gamma, double dt, int boost, int gizmo, int solver, double* const* flux)
+// ---- core-step stages -------------------------------------------------------
+int phd_gradients_packed(phd_handle* h, phd_plan* p, const phd_cells* c, int limiter,
+                         double* rec, int32_t rec_ld)
 {
-    int rc = phd_compute_gradients(h, c, f, limiter, grad, grad_ld);
-    if (rc) return rc;
-    rc = phd_update_ghost_gradients(h, c, boundary, lo, hi, grad, grad_ld);
-    if (rc) return rc;
-    rc = phd_face_flux(h, c, f, grad, grad_ld, gamma, 0.5 * dt, boost, gizmo, PHD_ORDER_LINEAR,
-                       solver, flux);
-    if (rc) return rc;
-    return phd_update_from_fluxes(h, c, f, flux, dt);
+    if (!h || check_cells(c) || check_plan(p, c) || check_rec(c->dim, rec, rec_ld)) return PHD_E_ARG;
+    return c->dim == 2 ? launch_grad<2>(h, p, c, limiter, rec, rec_ld)
+                       : launch_grad<3>(h, p, c, limiter, rec, rec_ld);
 }
 
-int phd_ghost_refresh_eos(phd_handle* h, const phd_cells* c, int boundary, const double* lo,
-                          const double* hi, double gamma)
+int phd_face_summands(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces* f,
+                      const double* rec, int32_t rec_ld, double gamma, double dt, int boost,
+                      int gizmo, int solver, double* const* flux)
+{
+    if (!h || check_cells(c) || check_faces(f) || check_plan(p, c) || check_plan_faces(p, f) ||
+        check_rec(c->dim, rec, rec_ld))
+        return PHD_E_ARG;
+    const int D = c->dim;
+    FaceArrays F = fa(flux, D + 2);
+    if (flux)
+        return D == 2 ? launch_face<2, 3>(h, p, c, f, rec, rec_ld, gamma, 0.5 * dt, dt, boost, gizmo, PHD_ORDER_LINEAR, solver, F)
+                      : launch_face<3, 3>(h, p, c, f, rec, rec_ld, gamma, 0.5 * dt, dt, boost, gizmo, PHD_ORDER_LINEAR, solver, F);
+    return D == 2 ? launch_face<2, 2>(h, p, c, f, rec, rec_ld, gamma, 0.5 * dt, dt, boost, gizmo, PHD_ORDER_LINEAR, solver, F)
+                  : launch_face<3, 2>(h, p, c, f, rec, rec_ld, gamma, 0.5 * dt, dt, boost, gizmo, PHD_ORDER_LINEAR, solver, F);
+}
+
+int phd_update_from_summands(phd_handle* h, phd_plan* p, const phd_cells* c)
+{
+    if (!h || check_cells(c) || check_plan(p, c)) return PHD_E_ARG;
+    FaceArrays F = fa(nullptr, 0);
+    return c->dim == 2 ? launch_update<2, 1>(h, p, c, nullptr, F, 0.0)
+                       : launch_update<3, 1>(h, p, c, nullptr, F, 0.0);
+}
+
+// ---- fused core step -----------------------------------------------------------
+int phd_core_step(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces* f,
+                  double* rec, int32_t rec_ld, int limiter, int boundary, const double* lo,
+                  const double* hi, double gamma, double dt, int boost, int gizmo, int solver,
+                  double* const* flux, int flags)
+{
+    if (!h || check_cells(c) || check_faces(f) || check_plan(p, c) || check_plan_faces(p, f) ||
+        check_rec(c->dim, rec, rec_ld))
+        return PHD_E_ARG;
+    int rc = 0;
+    if (!(flags & PHD_CORE_XP_FRESH)) rc = phd_pack_cells(h, p, c);
+    if (!rc) rc = phd_gradients_packed(h, p, c, limiter, rec, rec_ld);
+    if (!rc) rc = phd_update_ghost_gradients(h, p, c, boundary, lo, hi, rec, rec_ld);
+    if (!rc) rc = phd_face_summands(h, p, c, f, rec, rec_ld, gamma, dt, boost, gizmo, solver, flux);
+    if (!rc) rc = phd_update_from_summands(h, p, c);
+    return rc;
+}
+
+int phd_ghost_refresh_eos(phd_handle* h, phd_plan* p, const phd_cells* c, int boundary,
+                          const double* lo, const double* hi, double gamma)
 {
     if (!h || check_cells(c)) return PHD_E_ARG;
+    if (p && check_plan(p, c)) return PHD_E_ARG;
     if (c->n_cells == 0) return 0;
-    double l[3] = {0, 0, 0}, u[3] = {0, 0, 0};
-    if (boundary == PHD_BOUNDARY_REFLECTIVE) {
-        if (!lo || !hi) return PHD_E_ARG;
-        for (int k = 0; k < c->dim; k++) { l[k] = lo[k]; u[k] = hi[k]; }
-    } else if (boundary != PHD_BOUNDARY_PERIODIC) return PHD_E_MODE;
+    if (boundary != PHD_BOUNDARY_REFLECTIVE && boundary != PHD_BOUNDARY_PERIODIC) return PHD_E_MODE;
+    if (boundary == PHD_BOUNDARY_REFLECTIVE && (!lo || !hi)) return PHD_E_ARG;
+    double l[3], u[3];
+    bounds3(c->dim, boundary, lo, hi, l, u);
     const int refl = boundary == PHD_BOUNDARY_REFLECTIVE;
+    double* xp = p ? p->xp : nullptr;
     if (!c->map) {
-        // no ghosts: plain EOS
-        return phd_primitive_from_conservative(h, c, gamma);
+        int rc = phd_primitive_from_conservative(h, c, gamma);
+        if (rc || !p) return rc;
+        return phd_pack_cells(h, p, c);
     }
     if (c->dim == 2)
-        k_ghost_refresh_eos<2><<<nblocks(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, refl, l[0], l[1], l[2], u[0], u[1], u[2], gamma);
+        k_ghost_refresh_eos<2><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, refl, l[0], l[1], l[2], u[0], u[1], u[2], gamma, xp);
     else
-        k_ghost_refresh_eos<3><<<nblocks(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, refl, l[0], l[1], l[2], u[0], u[1], u[2], gamma);
+        k_ghost_refresh_eos<3><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, refl, l[0], l[1], l[2], u[0], u[1], u[2], gamma, xp);
     LAUNCHCHK();
     return 0;
 }

@@ -160,17 +160,23 @@ class DomainManager(object):
         if apply_boundary_condition:
             self.boundary_condition.update_fields(particles, self)
 
-    def update_ghost_gradients(self, particles, gradients):
+    def update_ghost_gradients(self, particles, gradients, mesh=None):
         """Ghost gradients from their images plus the boundary rule
         (domain_manager.pyx:819-837)."""
+        if mesh is not None:
+            self._mesh = mesh
         self.boundary_condition.update_gradients(particles, gradients, self)
 
     def _ghost_gradients(self, particles, gradients, boundary):
         lib, h = nat.flux(), nat.handle()
+        if getattr(self, "_mesh", None) is None:
+            raise RuntimeError("update_ghost_gradients needs the mesh whose plan holds the "
+                               "cell centroids (pass mesh=)")
+        plan = self._mesh.plan(particles)
         c = cells_view(particles)
         lo = (ctypes.c_double * 3)(*(self.xmin + [0.0] * (3 - self.dim)))
         hi = (ctypes.c_double * 3)(*(self.xmax + [0.0] * (3 - self.dim)))
-        nat.check(lib.phd_update_ghost_gradients(h, ctypes.byref(c), boundary,
+        nat.check(lib.phd_update_ghost_gradients(h, plan, ctypes.byref(c), boundary,
                                                  ctypes.cast(lo, ctypes.c_void_p),
                                                  ctypes.cast(hi, ctypes.c_void_p),
                                                  gradients.data.data_ptr(), gradients.ld),

@@ -79,6 +79,9 @@ class Mesh(object):
         self.faces = None
         self.neighbors = None
         self.n_threads = 0
+        self._plan = None
+        self._plan_key = None
+        self.version = 0
 
     def register_fields(self, particles):
         dim = len(particles.carray_named_groups["position"])
@@ -163,6 +166,27 @@ class Mesh(object):
             fc[f][:] = 0.0
         self.neighbors = FaceLists(torch.as_tensor(geo["nbr_off"], device=dev),
                                    torch.as_tensor(geo["nbr_face"], device=dev))
+        self.invalidate_plan()
+
+    # -- device plan ---------------------------------------------------------
+    def invalidate_plan(self):
+        """Call after changing face arrays, positions, dcom or volumes."""
+        self.version += 1
+        if self._plan is not None:
+            self._plan.close()
+        self._plan = None
+        self._plan_key = None
+
+    def plan(self, particles):
+        """The device plan of this mesh (built on first use after a rebuild)."""
+        key = (self.version, particles.get_carray_size(), self.faces.get_carray_size(),
+               particles.pointer("tag"), self.faces.pointer("area"))
+        if self._plan is None or self._plan_key != key:
+            if self._plan is not None:
+                self._plan.close()
+            self._plan = nat.Plan(cells_view(particles), faces_view(self))
+            self._plan_key = key
+        return self._plan.ptr
 
     # -- hot path (GPU) ------------------------------------------------------
     def assign_generator_velocities(self, particles, equation_state):
@@ -187,7 +211,8 @@ class Mesh(object):
         lib, h = nat.flux(), nat.handle()
         c, f = cells_view(particles), faces_view(self)
         fl = nat.ptr_array(riemann.fluxes.group("conservative"))
-        nat.check(lib.phd_update_from_fluxes(h, ctypes.byref(c), ctypes.byref(f), fl, float(dt)),
+        nat.check(lib.phd_update_from_fluxes(h, self.plan(particles), ctypes.byref(c),
+                                             ctypes.byref(f), fl, float(dt)),
                   "update_from_fluxes")
         if check_errors:
             nat.raise_device_fault(h)

@@ -180,6 +180,7 @@ class FrozenStep(object):
         self.gizmo = int(recon.gizmo_limiter)
         self.solver = problem.riemann.solver
         self.boost = int(problem.riemann.boost)
+        self.plan = mesh.plan(pc)
 
     def prologue(self):
         """Generator and face velocities (mesh.pyx:363-477)."""
@@ -189,19 +190,27 @@ class FrozenStep(object):
     def time_step(self):
         return self.p.riemann.compute_time_step(self.p.particles, self.p.eos)
 
-    def run(self, dt, h=None):
+    def run(self, dt, h=None, write_flux=True, xp_fresh=False):
+        """One core step.  write_flux=False skips writing riemann.fluxes (the
+        update uses the (dt*area)*flux rows either way); xp_fresh=True when
+        the previous epilogue refreshed the plan's packed cell rows."""
         h = nat.handle() if h is None else h
-        rc = self.lib.phd_core_step(h, ctypes.byref(self.cells), ctypes.byref(self.faces),
-                                    self.grad.data.data_ptr(), self.grad.ld, self.limiter,
-                                    self.boundary, ctypes.cast(self.lo, ctypes.c_void_p),
+        rc = self.lib.phd_core_step(h, self.plan, ctypes.byref(self.cells),
+                                    ctypes.byref(self.faces), self.grad.data.data_ptr(),
+                                    self.grad.ld, self.limiter, self.boundary,
+                                    ctypes.cast(self.lo, ctypes.c_void_p),
                                     ctypes.cast(self.hi, ctypes.c_void_p), self.gamma, float(dt),
-                                    self.boost, self.gizmo, self.solver, self.flux_ptrs)
+                                    self.boost, self.gizmo, self.solver,
+                                    self.flux_ptrs if write_flux else None,
+                                    nat.CORE_XP_FRESH if xp_fresh else 0)
         nat.check(rc, "phd_core_step")
 
     def epilogue(self, h=None):
-        """Ghost conserved refresh with the boundary rule + EOS, all cells."""
+        """Ghost conserved refresh with the boundary rule + EOS, all cells;
+        refreshes the plan's packed cell rows."""
         h = nat.handle() if h is None else h
-        rc = self.lib.phd_ghost_refresh_eos(h, ctypes.byref(self.cells), self.boundary,
+        rc = self.lib.phd_ghost_refresh_eos(h, self.plan, ctypes.byref(self.cells),
+                                            self.boundary,
                                             ctypes.cast(self.lo, ctypes.c_void_p),
                                             ctypes.cast(self.hi, ctypes.c_void_p), self.gamma)
         nat.check(rc, "phd_ghost_refresh_eos")

@@ -75,7 +75,8 @@ class ReconstructionBase(object):
         grad, ld = self._grad_ptr()
         L = nat.ptr_array(self._left.group("primitive"))
         R = nat.ptr_array(self._right.group("primitive"))
-        nat.check(lib.phd_compute_states(h, ctypes.byref(c), ctypes.byref(f), grad, ld, gamma, dt,
+        nat.check(lib.phd_compute_states(h, mesh.plan(particles), ctypes.byref(c),
+                                         ctypes.byref(f), grad, ld, gamma, dt,
                                          int(boost), int(self._gizmo()), self.order, stages, L, R),
                   "compute_states")
 
@@ -150,7 +151,7 @@ class PieceWiseLinear(ReconstructionBase):
             raise RuntimeError("Reconstruction did not set fields to reconstruct!")
         self._state_containers()
         self.grad = GradientContainer(self.grad_field_names, self.dim,
-                                      device=self._left.device)
+                                      nat.record_ld(self.dim), device=self._left.device)
         self.reconstruct_grads = {g: "double" for g in self.grad.names}
         self.reconstruct_grad_groups = self.grad.carray_named_groups
 
@@ -165,13 +166,13 @@ class PieceWiseLinear(ReconstructionBase):
         transfer (739)."""
         self.grad.resize(particles.get_carray_size())
         lib, h = nat.flux(), nat.handle()
-        c, f = cells_view(particles), faces_view(mesh)
-        nat.check(lib.phd_compute_gradients(h, ctypes.byref(c), ctypes.byref(f),
+        c = cells_view(particles)
+        nat.check(lib.phd_compute_gradients(h, mesh.plan(particles), ctypes.byref(c),
                                             self.slope_limiter, self.grad.data.data_ptr(),
                                             self.grad.ld), "compute_gradients")
         if check_errors:
             nat.raise_device_fault(h)
-        domain_manager.update_ghost_gradients(particles, self.grad)
+        domain_manager.update_ghost_gradients(particles, self.grad, mesh)
 
     def add_spatial(self, particles, mesh, gamma, domain_manager, dt, boost):
         self._materialize()

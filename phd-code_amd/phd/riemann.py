@@ -48,7 +48,8 @@ class RiemannBase(object):
             pparticles, pmesh, pgamma, dt, boost = pending
             c = cells_view(pparticles)
             grad, ld = reconstruction._grad_ptr()
-            rc = lib.phd_face_flux(h, ctypes.byref(c), ctypes.byref(f), grad, ld, pgamma, dt,
+            rc = lib.phd_face_flux(h, pmesh.plan(pparticles), ctypes.byref(c), ctypes.byref(f),
+                                   grad, ld, pgamma, dt,
                                    int(boost), int(reconstruction._gizmo()),
                                    reconstruction.order, self.solver, fl)
             nat.check(rc, "compute_fluxes")
