@@ -1,0 +1,120 @@
+"""CPU checks of the C ABI boundary and of the host-side plugin logic.
+
+  - libphd_flux.so loads without a GPU and exports every function that
+    include/phd_flux.h declares (and libphd_mesh.so those of phd_mesh.h);
+  - the ctypes layouts match the C structs;
+  - the plugin classes keep the reference's constructors, field schema and
+    error behaviour (RuntimeError messages of reconstruction.pyx,
+    riemann.pyx, integrate.py); compute calls without a GPU raise
+    NativeUnavailable instead of falling back to the CPU.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import phd
+from phd import _native as nat
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(\w+)\s*\(", txt, re.M)))
+
+
+def test_flux_library_exports_every_declared_symbol():
+    lib = nat.load_flux_library()
+    names = _declared("phd_flux.h")
+    assert "phd_core_step" in names and "phd_plan_create" in names
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(nat.flux_symbols())
+    assert lib.phd_abi_version() == nat.PHD_ABI_VERSION
+    assert lib.phd_cell_record_ld(3) == nat.record_ld(3) == 24
+    assert lib.phd_cell_record_ld(2) == nat.record_ld(2) == 14
+    assert lib.phd_error_string(nat.PHD_F_MASS).decode() == "Mass less than zero in flux update"
+
+
+def test_mesh_library_exports_every_declared_symbol():
+    lib = nat.mesh_lib()
+    for name in _declared("phd_mesh.h"):
+        assert hasattr(lib, name), name
+
+
+def test_struct_layouts_match_header():
+    # phd_cells: 2 int32 then 26 pointers (tag..w); phd_faces: int32 + 13 pointers
+    assert ctypes.sizeof(nat.Cells) == 8 + 8 * (3 + 3 + 3 + 1 + 5 + 5 + 3)
+    assert nat.Cells.tag.offset == 8
+    assert ctypes.sizeof(nat.Faces) == 8 + 8 * (1 + 3 + 3 + 3 + 2 + 2)
+    assert nat.Faces.area.offset == 8
+
+
+def test_container_semantics():
+    pc = phd.HydroParticleCreator(num=5, dim=3, device="cpu")
+    g = pc.carray_named_groups
+    assert g["primitive"] == ["density", "velocity-x", "velocity-y", "velocity-z", "pressure"]
+    assert g["conservative"] == ["mass", "momentum-x", "momentum-y", "momentum-z", "energy"]
+    assert pc["tag"].dtype == torch.int8 and pc["ids"].dtype == torch.int32
+    pc["density"][:] = torch.arange(5, dtype=torch.float64)
+    pc["tag"][:] = torch.tensor([0, 1, 0, 1, 0], dtype=torch.int8)
+    pc.resize(7)
+    assert pc.get_carray_size() == 7 and pc["density"][:5].tolist() == [0, 1, 2, 3, 4]
+    pc.resize(5)
+    ex = pc.extract_items([4, 0])
+    assert ex["density"].tolist() == [4.0, 0.0]
+    pc.append_container(ex)
+    assert pc.get_carray_size() == 7 and pc["density"][5:].tolist() == [4.0, 0.0]
+    pc.remove_tagged_particles(phd.ParticleTAGS.Ghost)
+    assert pc["density"].tolist() == [0.0, 2.0, 4.0, 4.0, 0.0]
+
+
+def test_plugin_constructors_and_errors():
+    with pytest.raises(RuntimeError, match="Unrecognized limiter"):
+        phd.PieceWiseLinear(limiter="minmod")
+    r = phd.PieceWiseLinear()
+    with pytest.raises(RuntimeError, match="did not set fields"):
+        r.initialize()
+    bad = phd.CarrayContainer(1, {"wrong_type": "int"}, device="cpu")
+    with pytest.raises(RuntimeError, match="Missing fields"):
+        r.add_fields(bad)
+    h = phd.HLLC()
+    assert h.cfl == 0.5 and h.boost is True
+    with pytest.raises(RuntimeError, match="did not set fields for flux"):
+        h.initialize()
+    ex = phd.Exact(cfl=0.1)
+    assert ex.cfl == 0.5 and ex.boost is True
+    integ = phd.MovingMeshMUSCLHancock()
+    with pytest.raises(RuntimeError, match="only accepts class type"):
+        integ.set_riemann(phd.IdealGas())
+    with pytest.raises(RuntimeError, match="Not all setters defined"):
+        integ.initialize()
+    with pytest.raises(RuntimeError, match="Wrong dimension"):
+        phd.DomainManager([0.0], [1.0])
+
+
+def test_gradient_container_named_groups():
+    r = phd.PieceWiseLinear()
+    pc = phd.HydroParticleCreator(num=3, dim=2, device="cpu")
+    r.add_fields(pc)
+    r.initialize()
+    g = r.grad
+    assert g.carray_named_groups["velocity"] == ["velocity-x_x", "velocity-x_y",
+                                                 "velocity-y_x", "velocity-y_y"]
+    assert g.column["density_x"] == 4 and g.ld == 14
+    g.resize(3)
+    g["pressure_y"] = torch.tensor([1.0, 2.0, 3.0], dtype=torch.float64)
+    assert g.rows()[:, 7].tolist() == [1.0, 2.0, 3.0]
+
+
+@pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-GPU behaviour")
+def test_compute_without_gpu_raises():
+    pc = phd.HydroParticleCreator(num=4, dim=2, device="cpu")
+    pc.register_carray(4, "volume", "double")
+    with pytest.raises(phd.NativeUnavailable):
+        phd.IdealGas().conservative_from_primitive(pc)
