@@ -30,6 +30,13 @@ for p in (os.path.join(ROOT, "phd-code_amd"), ROOT):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+CSRC = os.path.join(ROOT, "phd-code_amd", "phd", "csrc")
+PROFILES = os.path.join(ROOT, "profiles")
+# bench stage -> rocprofv3 kernel name of the 3D kernel it launches
+KERNEL_NAMES = {"gradients": "k_grad_sell<3, 0>", "face_flux": "k_face<3, 1, 1, 2>",
+                "update": "k_update_sell<3>", "ghost_gradients": "k_ghost_rec<3>",
+                "epilogue": "k_ghost_refresh_eos<3>"}
+COPY_KERNEL = "k_pack_xp<3>"
 
 
 def algorithmic_bytes(dim, nr, ng, nf):
@@ -60,6 +67,84 @@ def algorithmic_bytes(dim, nr, ng, nf):
     return b
 
 
+def source_hash():
+    """sha256 over the HIP sources and the C header: a traffic record is
+    used only for the kernels it was measured on."""
+    import hashlib
+    hsh = hashlib.sha256()
+    files = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
+                   if f.endswith((".hip", ".h")))
+    files.append(os.path.join(ROOT, "include", "phd_flux.h"))
+    for fn in files:
+        with open(fn, "rb") as fh:
+            hsh.update(os.path.basename(fn).encode() + b"\0" + fh.read())
+    return hsh.hexdigest()
+
+
+def traffic_path(n):
+    return os.path.join(PROFILES, "traffic_sedov3d_%d.json" % n)
+
+
+def _pmc_means(csv_path, counter):
+    """Mean per-dispatch value of one counter for each kernel in a
+    rocprofv3 --pmc counter_collection.csv."""
+    import csv
+    import re
+    vals = {}
+    with open(csv_path) as fh:
+        for r in csv.DictReader(fh):
+            if r["Counter_Name"] != counter:
+                continue
+            m = re.search(r"(k_\w+<[^>]*>)", r["Kernel_Name"])
+            if m:
+                vals.setdefault(m.group(1), []).append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in vals.items()}
+
+
+def write_traffic(fetch_csv, write_csv, n, command):
+    """Turn two rocprofv3 --pmc runs of this benchmark (FETCH_SIZE, then
+    WRITE_SIZE; KiB per dispatch) into HBM bytes per launch per kernel.
+    Each counter is scaled by what it reads for k_pack_xp, a copy kernel of
+    known size: it reads (D+2)+D doubles per cell and writes one 64-byte row
+    per cell, all cells."""
+    fetch = _pmc_means(fetch_csv, "FETCH_SIZE")
+    write = _pmc_means(write_csv, "WRITE_SIZE")
+    from phd.mesh import build_voronoi
+    from phd.problems import jittered_lattice
+    g = build_voronoi(3, jittered_lattice(n, 3, 0), [0.0] * 3, [1.0] * 3, 0)
+    na_bytes = (g["n_real"] + g["n_ghost"]) * 64
+    f_cal = na_bytes / (fetch[COPY_KERNEL] * 1024.0)
+    w_cal = na_bytes / (write[COPY_KERNEL] * 1024.0)
+    kernels = {}
+    for stage, kn in KERNEL_NAMES.items():
+        if kn in fetch and kn in write:
+            rd = fetch[kn] * 1024.0 * f_cal
+            wr = write[kn] * 1024.0 * w_cal
+            kernels[stage] = {"kernel": kn, "fetch_kib": fetch[kn], "write_kib": write[kn],
+                              "read_bytes": rd, "write_bytes": wr, "traffic_bytes": rd + wr}
+    rec = {"workload": "sedov3d n=%d" % n, "source_sha256": source_hash(),
+           "copy_kernel": COPY_KERNEL, "copy_known_bytes": na_bytes,
+           "fetch_scale": f_cal, "write_scale": w_cal, "kernels": kernels,
+           "command": command}
+    os.makedirs(PROFILES, exist_ok=True)
+    with open(traffic_path(n), "w") as fh:
+        json.dump(rec, fh, indent=1)
+    return rec
+
+
+def measured_traffic(n, stage):
+    """HBM bytes per launch of a stage's kernel from the committed traffic
+    record, or None when there is none for the current sources."""
+    try:
+        with open(traffic_path(n)) as fh:
+            rec = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if rec.get("source_sha256") != source_hash() or stage not in rec.get("kernels", {}):
+        return None
+    return rec["kernels"][stage]["traffic_bytes"]
+
+
 def cpu_baseline(problem, dt):
     """One full core step of the same workload through the oracle, one core."""
     from phd import problems
@@ -84,7 +169,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write the traffic record from two rocprofv3 --pmc runs of this "
+                         "benchmark (FETCH_SIZE, WRITE_SIZE) and exit; no GPU needed")
+    ap.add_argument("--traffic-command", default="",
+                    help="the profiling command, stored in the traffic record")
     args = ap.parse_args()
+    if args.traffic_from:
+        rec = write_traffic(args.traffic_from[0], args.traffic_from[1], args.n,
+                            args.traffic_command)
+        print(json.dumps(rec, indent=1))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the face-flux pipeline here is the single-GPU "
               "kernel at its real shape (multi-GPU sharding is a design, not part of this "
@@ -180,9 +275,11 @@ def main():
     B = algorithmic_bytes(prob.dim, nr, ng, nf)
     dom = max(("gradients", "face_flux", "update"), key=lambda n: kern_ms[n])
     achieved = B[dom] / (kern_ms[dom] * 1e-3) / 1e9
+    traffic = measured_traffic(args.n, dom)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": dom}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None if traffic is None else round(traffic),
+                "kernel": dom, "algorithmic_bytes": B[dom]}
 
     result = {
         "metric": "Voronoi face-flux updates/s FP64 (3D Sedov 2M cells) + %HBM roofline; "

@@ -384,8 +384,76 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_pack_xp(phd_cells c, double* xp)
 }
 
 // ---------------------------------------------------------------------------
+// Division by one divisor shared by many numerators.  hipcc lowers a / b on
+// gfx950 to div_scale, rcp, two Newton steps on the reciprocal, q0 = a*r,
+// e = fma(-b, q0, a), div_fmas, div_fixup.  For |a|, |b| in [2^-200, 2^200]
+// div_scale returns its operand unchanged with VCC clear (no operand or
+// quotient is near the overflow/underflow limits), div_fmas reduces to
+// fma(e, r, q0) and div_fixup to the identity, so the steps below give the
+// bits of a / b while the reciprocal and its two Newton steps are computed
+// once per divisor.  The residual is formed as b*q0 - a and subtracted, which
+// is the same exact value with the opposite sign; for b > 0 that makes a zero
+// numerator come out as the correctly signed zero.  Callers check the range
+// and b > 0 and use the ordinary division otherwise.
+__device__ __forceinline__ double rcp_newton(double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    return fma(r, e, r);
+}
+
+__device__ __forceinline__ double div_shared(double a, double b, double r)
+{
+    const double q = a * r;
+    const double e = fma(b, q, -a);
+    return fma(-e, r, q);
+}
+
+// |x| in [2^-200, 2^200] or x == 0; false for inf and NaN (non-short-circuit
+// operators: the test compiles to compares and mask logic, no branches)
+__device__ __forceinline__ bool div_operand_ok(double x)
+{
+    const double ax = fabs(x);
+    return (ax <= 0x1p200) & ((ax >= 0x1p-200) | (x == 0.0));
+}
+
+// one entry of a sliced list: neighbour id (INCOMING bit cleared), face
+// centroid, face area
+template <int D> struct SlotEnt {
+    int j;
+    double com[3];
+    double area;
+};
+
+template <int D>
+__device__ __forceinline__ SlotEnt<D> load_slot(const int32_t* sj, const double* sx,
+                                                const double* sy, const double* sz,
+                                                const double* sa, long long slot)
+{
+    SlotEnt<D> s;
+    s.j = __builtin_nontemporal_load(sj + slot) & 0x7fffffff;
+    s.com[0] = __builtin_nontemporal_load(sx + slot);
+    s.com[1] = __builtin_nontemporal_load(sy + slot);
+    s.com[2] = D == 3 ? __builtin_nontemporal_load(sz + slot) : 0.0;
+    s.area = __builtin_nontemporal_load(sa + slot);
+    return s;
+}
+
+// packed [position | primitive] row of cell j
+__device__ __forceinline__ void load_row(const double* xp, int j, double* r)
+{
+#pragma unroll
+    for (int q = 0; q < XPW / 2; q++) ld2(xp + (size_t)j * XPW + 2 * q, r[2 * q], r[2 * q + 1]);
+}
+
 // gradients + slope limiter (reconstruction.pyx:563-737), one thread per
-// real cell; writes the cell record row [prim | alpha*df | centroid].
+// real cell; writes the cell record row [prim | alpha*df | centroid].  The
+// neighbour loop walks the cell's sliced list (sbase/sj/sgeo, see phd_plan),
+// so the per-entry face data is streamed with one coalesced load per field
+// per 16 lanes and only the packed [position | primitive] row of each
+// neighbour is gathered.
 //
 // The AREPO limiter (683-703) takes alpha_n = min(1, min_faces fmax(psi,0))
 // with psi = (phi_max - phi_i)/dphi for dphi > 0 and (phi_min - phi_i)/dphi
@@ -393,152 +461,6 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_pack_xp(phd_cells c, double* xp)
 // correctly rounded quotient is monotone in its divisor, so the minimum over
 // faces is reached at the largest positive dphi and at the most negative
 // dphi: two divisions per field instead of one per face, same bits.
-template <int D, int LIM>
-__global__ __launch_bounds__(PHD_BLOCK) void k_grad(int n_cells, const int8_t* __restrict__ tag,
-                                                    const double* __restrict__ geo,
-                                                    const double* __restrict__ xp,
-                                                    const int2* __restrict__ csr,
-                                                    const int32_t* __restrict__ off,
-                                                    const double* __restrict__ frec,
-                                                    double* __restrict__ rec, int ld)
-{
-    using Y = Lay<D>;
-    constexpr int NF = Y::NF;
-    long long ii = xcd_index(n_cells);
-    if (ii >= n_cells) return;
-    const int i = (int)ii;
-    if (tag[i] != 0) return;
-
-    double cx[D], vol, xi[D], pv[NF];
-    {
-        double g[GEO];
-        ld2(geo + (size_t)i * GEO, g[0], g[1]);
-        ld2(geo + (size_t)i * GEO + 2, g[2], g[3]);
-#pragma unroll
-        for (int k = 0; k < D; k++) cx[k] = g[k];
-        vol = g[3];
-        double r[XPW];
-#pragma unroll
-        for (int q = 0; q < XPW / 2; q++) ld2(xp + (size_t)i * XPW + 2 * q, r[2 * q], r[2 * q + 1]);
-#pragma unroll
-        for (int k = 0; k < D; k++) xi[k] = r[k];
-#pragma unroll
-        for (int n = 0; n < NF; n++) pv[n] = r[D + n];
-    }
-    double pmax[NF], pmin[NF], df[NF * D];
-#pragma unroll
-    for (int n = 0; n < NF; n++) {
-        pmax[n] = pmin[n] = pv[n];
-#pragma unroll
-        for (int k = 0; k < D; k++) df[n * D + k] = 0.0;
-    }
-    const int beg = off[i], end = off[i + 1];
-    for (int e = beg; e < end; e++) {
-        const int2 en = csr[e];
-        const int j = en.y & 0x7fffffff;
-        if (j == INVALID) continue;
-        double com[3], area;
-        ld2(frec + (size_t)en.x * FREC, com[0], com[1]);
-        ld2(frec + (size_t)en.x * FREC + 2, com[2], area);
-        double r[XPW];
-#pragma unroll
-        for (int q = 0; q < XPW / 2; q++) ld2(xp + (size_t)j * XPW + 2 * q, r[2 * q], r[2 * q + 1]);
-        double cfx[D], dr[D], rr = 0.0;
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            const double xj = r[k];
-            cfx[k] = com[k] - 0.5 * (xi[k] + xj);
-            dr[k] = xi[k] - xj;
-            rr += dr[k] * dr[k];
-        }
-        rr = sqrt(rr);
-        const double rv = rr * vol;
-#pragma unroll
-        for (int n = 0; n < NF; n++) {
-            const double pj = r[D + n];
-            pmax[n] = fmax(pmax[n], pj);
-            pmin[n] = fmin(pmin[n], pj);
-            const double d_dif = pj - pv[n];
-            const double d_sum = pj + pv[n];
-#pragma unroll
-            for (int k = 0; k < D; k++)
-                df[n * D + k] += area * (d_dif * cfx[k] - 0.5 * d_sum * dr[k]) / rv;
-        }
-    }
-
-    double alpha[NF];
-#pragma unroll
-    for (int n = 0; n < NF; n++) alpha[n] = 1.0;
-    if (LIM == PHD_LIMITER_AREPO) {
-        double dpos[NF], dneg[NF];
-#pragma unroll
-        for (int n = 0; n < NF; n++) { dpos[n] = 0.0; dneg[n] = 0.0; }
-        for (int e = beg; e < end; e++) {
-            const int2 en = csr[e];
-            double com[3], area;
-            ld2(frec + (size_t)en.x * FREC, com[0], com[1]);
-            ld2(frec + (size_t)en.x * FREC + 2, com[2], area);
-            double rel[D];
-#pragma unroll
-            for (int k = 0; k < D; k++) rel[k] = com[k] - cx[k];
-#pragma unroll
-            for (int n = 0; n < NF; n++) {
-                double dphi = 0.0;
-#pragma unroll
-                for (int k = 0; k < D; k++) dphi += df[n * D + k] * rel[k];
-                if (dphi > 0) dpos[n] = fmax(dpos[n], dphi);
-                else if (dphi < 0) dneg[n] = fmin(dneg[n], dphi);
-            }
-        }
-#pragma unroll
-        for (int n = 0; n < NF; n++) {
-            if (dpos[n] > 0) alpha[n] = fmin(alpha[n], fmax((pmax[n] - pv[n]) / dpos[n], 0.));
-            if (dneg[n] < 0) alpha[n] = fmin(alpha[n], fmax((pmin[n] - pv[n]) / dneg[n], 0.));
-        }
-    } else {
-        // TESS limiter (705-731); as in the reference only alpha[0] changes
-        for (int n = 0; n < NF; n++) {
-            for (int e = beg; e < end; e++) {
-                const int2 en = csr[e];
-                const int j = en.y & 0x7fffffff;
-                if (j == INVALID) continue;
-                double com[3], area;
-                ld2(frec + (size_t)en.x * FREC, com[0], com[1]);
-                ld2(frec + (size_t)en.x * FREC + 2, com[2], area);
-                double dphi = 0.0;
-#pragma unroll
-                for (int k = 0; k < D; k++) dphi += df[n * D + k] * (com[k] - cx[k]);
-                double psi;
-                if (dphi > 0.0 || dphi < 0.0) {
-                    double q = (xp[(size_t)j * XPW + D + n] - pv[n]) / dphi;
-                    psi = (0. > q) ? 0. : q;
-                } else {
-                    psi = 1.0;
-                }
-                alpha[0] = fmin(alpha[0], fmax(psi, 0.));
-            }
-        }
-    }
-    double* row = rec + (size_t)i * ld;
-    double out[Y::LD];
-#pragma unroll
-    for (int n = 0; n < NF; n++) out[Y::P0 + n] = pv[n];
-#pragma unroll
-    for (int n = 0; n < NF; n++)
-#pragma unroll
-        for (int k = 0; k < D; k++) out[Y::G0 + n * D + k] = alpha[n] * df[n * D + k];
-#pragma unroll
-    for (int k = 0; k < D; k++) out[Y::C0 + k] = cx[k];
-    if (Y::LD > Y::REC) out[Y::LD - 1] = 0.0;
-#pragma unroll
-    for (int q = 0; q < Y::LD / 2; q++)
-        reinterpret_cast<double2*>(row)[q] = make_double2(out[2 * q], out[2 * q + 1]);
-}
-
-// gradients + limiter over the sliced lists: the per-entry face data is
-// streamed (one coalesced load per field per 16 lanes), only the packed
-// [position | primitive] row of each neighbour is gathered.  Same arithmetic,
-// same order as k_grad.
 template <int D, int LIM>
 __global__ __launch_bounds__(PHD_BLOCK) void k_grad_sell(int n_cells, const int8_t* __restrict__ tag,
                                                          const double* __restrict__ geo,
@@ -585,17 +507,13 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_grad_sell(int n_cells, const int8
     const int deg = off[i + 1] - off[i];
     const long long base = sbase[i / SLICE] + (i % SLICE);
     for (int e = 0; e < deg; e++) {
-        const long long slot = base + (long long)e * SLICE;
-        const int j = __builtin_nontemporal_load(sj + slot) & 0x7fffffff;
+        const SlotEnt<D> cur = load_slot<D>(sj, sx, sy, sz, sa, base + (long long)e * SLICE);
+        const int j = cur.j;
         if (j == INVALID) continue;
-        double com[3];
-        com[0] = __builtin_nontemporal_load(sx + slot);
-        com[1] = __builtin_nontemporal_load(sy + slot);
-        com[2] = D == 3 ? __builtin_nontemporal_load(sz + slot) : 0.0;
-        const double area = __builtin_nontemporal_load(sa + slot);
+        const double* com = cur.com;
+        const double area = cur.area;
         double r[XPW];
-#pragma unroll
-        for (int q = 0; q < XPW / 2; q++) ld2(xp + (size_t)j * XPW + 2 * q, r[2 * q], r[2 * q + 1]);
+        load_row(xp, j, r);
         double cfx[D], dr[D], rr = 0.0;
 #pragma unroll
         for (int k = 0; k < D; k++) {
@@ -606,6 +524,9 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_grad_sell(int n_cells, const int8
         }
         rr = sqrt(rr);
         const double rv = rr * vol;
+        // Springel (2010) eq. 21 summand, area*(...)/(r*vol) (677-681)
+        double num[NF * D];
+        bool shared = div_operand_ok(rv) & (rv > 0.0);
 #pragma unroll
         for (int n = 0; n < NF; n++) {
             const double pj = r[D + n];
@@ -614,8 +535,18 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_grad_sell(int n_cells, const int8
             const double d_dif = pj - pv[n];
             const double d_sum = pj + pv[n];
 #pragma unroll
-            for (int k = 0; k < D; k++)
-                df[n * D + k] += area * (d_dif * cfx[k] - 0.5 * d_sum * dr[k]) / rv;
+            for (int k = 0; k < D; k++) {
+                num[n * D + k] = area * (d_dif * cfx[k] - 0.5 * d_sum * dr[k]);
+                shared &= div_operand_ok(num[n * D + k]);
+            }
+        }
+        if (shared) {
+            const double rcp = rcp_newton(rv);
+#pragma unroll
+            for (int q = 0; q < NF * D; q++) df[q] += div_shared(num[q], rv, rcp);
+        } else {
+#pragma unroll
+            for (int q = 0; q < NF * D; q++) df[q] += num[q] / rv;
         }
     }
 
@@ -809,7 +740,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_states(phd_cells c, phd_faces f,
 // Riemann fluxes from materialised states (riemann.pyx:74-98)
 template <int D, int SOLVER>
 __global__ __launch_bounds__(PHD_BLOCK) void k_riemann(phd_faces f, FaceArrays left,
-                                                       FaceArrays right, double gamma, int boost,
+                                                       FaceArrays right, Gas gas, int boost,
                                                        FaceArrays flux, unsigned long long* err)
 {
     constexpr int NF = D + 2;
@@ -820,7 +751,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_riemann(phd_faces f, FaceArrays l
     for (int q = 0; q < NF; q++) { L[q] = left.q[q][m]; R[q] = right.q[q][m]; }
 #pragma unroll
     for (int k = 0; k < D; k++) { n[k] = f.normal[k][m]; fv[k] = f.velocity[k][m]; }
-    if (!solve_face<D, SOLVER>(L, R, n, fv, gamma, boost, F)) {
+    if (!solve_face<D, SOLVER>(L, R, n, fv, gas, boost, F)) {
         atomicMin(&err[2], (unsigned long long)m);
         return;
     }
@@ -835,7 +766,7 @@ template <int D, int ORDER, int SOLVER, int OUT>
 __global__ __launch_bounds__(PHD_BLOCK) void k_face(phd_cells c, phd_faces f,
                                                     const double* __restrict__ frec,
                                                     const double* __restrict__ rec, int ld,
-                                                    double gamma, double dt_half, double dt_full,
+                                                    Gas gas, double dt_half, double dt_full,
                                                     int boost, int gizmo, FaceArrays flux,
                                                     FaceArrays sg,
                                                     const int32_t* __restrict__ fslot,
@@ -849,9 +780,9 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_face(phd_cells c, phd_faces f,
     load_face_rec<D>(frec, m, com, area, n, diff);
 #pragma unroll
     for (int k = 0; k < D; k++) fv[k] = f.velocity[k][m];
-    face_states<D, ORDER>(c, rec, ld, i, j, com, diff, fv, gamma, dt_half, boost, gizmo,
+    face_states<D, ORDER>(c, rec, ld, i, j, com, diff, fv, gas.gamma, dt_half, boost, gizmo,
                           PHD_STAGE_SPATIAL | PHD_STAGE_TEMPORAL, L, R);
-    if (!solve_face<D, SOLVER>(L, R, n, fv, gamma, boost, F)) {
+    if (!solve_face<D, SOLVER>(L, R, n, fv, gas, boost, F)) {
         atomicMin(&err[2], (unsigned long long)m);
         return;
     }
@@ -1154,7 +1085,7 @@ int launch_face_o(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_face
 {
     const unsigned nb = xcd_grid(f->n_faces);
     switch (solver) {
-#define FACE(S) k_face<D, ORDER, S, OUT><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, p->face_rec, rec, ld, gamma, dt_half, dt_full, boost, gizmo, flux, slot_g(p), p->fslot, h->d_err)
+#define FACE(S) k_face<D, ORDER, S, OUT><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, p->face_rec, rec, ld, make_gas(gamma), dt_half, dt_full, boost, gizmo, flux, slot_g(p), p->fslot, h->d_err)
     case PHD_SOLVER_HLL: FACE(PHD_SOLVER_HLL); break;
     case PHD_SOLVER_HLLC: FACE(PHD_SOLVER_HLLC); break;
     case PHD_SOLVER_EXACT: FACE(PHD_SOLVER_EXACT); break;
@@ -1545,7 +1476,7 @@ int phd_compute_fluxes(phd_handle* h, const phd_faces* f, int dim, double* const
     const int nf = dim + 2;
     FaceArrays L = fa(left, nf), R = fa(right, nf), F = fa(flux, nf);
     const unsigned nb = xcd_grid(f->n_faces);
-#define RIE(DD, SS) k_riemann<DD, SS><<<nb, PHD_BLOCK, 0, h->stream>>>(*f, L, R, gamma, boost, F, h->d_err)
+#define RIE(DD, SS) k_riemann<DD, SS><<<nb, PHD_BLOCK, 0, h->stream>>>(*f, L, R, make_gas(gamma), boost, F, h->d_err)
     if (dim == 2) {
         if (solver == PHD_SOLVER_HLL) RIE(2, PHD_SOLVER_HLL);
         else if (solver == PHD_SOLVER_HLLC) RIE(2, PHD_SOLVER_HLLC);

@@ -9,12 +9,47 @@
 
 namespace phd {
 
+// Gamma-only subexpressions of the HLL-family solvers, evaluated once on the
+// host with the reference's expressions.  Division and subtraction are
+// correctly rounded on both sides, so each constant has the same bits as the
+// per-face evaluation in riemann.pyx.
+struct Gas {
+    double gamma;
+    double z;        // (gamma - 1.)/(2.*gamma)        riemann.pyx:452
+    double inv_z;    // 1./z                            riemann.pyx:460-461
+    double gp1_2g;   // (gamma+1.)/(2.*gamma)           riemann.pyx:488,497
+    double gm1_gp1;  // (gamma - 1.)/(gamma + 1.)       riemann.pyx:470-471
+};
+
+inline Gas make_gas(double gamma)
+{
+    Gas g;
+    g.gamma = gamma;
+    g.z = (gamma - 1.) / (2. * gamma);
+    g.inv_z = 1. / g.z;
+    g.gp1_2g = (gamma + 1.) / (2. * gamma);
+    g.gm1_gp1 = (gamma - 1.) / (gamma + 1.);
+    return g;
+}
+
+// pow(1, y) is exactly 1 for every y (C99 F.9.4.4), so skipping the call for
+// a base of exactly one changes no bit.  In a uniform region every TRRS base
+// is exactly one (equal pressures, equal velocities), and a wave whose lanes
+// all skip never executes the pow sequence.
+__device__ __forceinline__ double pow_b1(double x, double y)
+{
+    double r = 1.0;
+    if (__builtin_expect(x != 1.0, 0)) r = pow(x, y);
+    return r;
+}
+
 // HLL.get_waves (riemann.pyx:375-504): Toro PVRS / TRRS / TSRS p* estimate
 // and Toro eq. 10.68-10.70 wave speeds.
 __device__ __forceinline__ void get_waves(double dl, double ul, double pl, double dr, double ur,
-                                          double pr, double gamma, double& sl, double& sc,
+                                          double pr, const Gas& g, double& sl, double& sc,
                                           double& sr)
 {
+    const double gamma = g.gamma;
     const double Q = 2., c_floor = 1.0E-10;
     double cl = fmax(sqrt(gamma * pl / dl), c_floor);
     double cr = fmax(sqrt(gamma * pr / dr), c_floor);
@@ -26,25 +61,24 @@ __device__ __forceinline__ void get_waves(double dl, double ul, double pl, doubl
     if (((p_max / p_min) < Q) && ((p_min < p_star) && (p_star < p_max))) {
         // PVRS: p_star kept (u_star at 445 is not used further)
     } else if (p_star <= p_min) {
-        double z = (gamma - 1.) / (2. * gamma);
-        double plr = pow(pl / pr, z);
+        double plr = pow_b1(pl / pr, g.z);
         double u_star = (plr * ul / cl + ur / cr + 2. * (plr - 1.) / (gamma - 1.)) /
                         (plr / cl + 1. / cr);
-        p_star = .5 * pl * pow(1. + (gamma - 1.) * (ul - u_star) / (2. * cl), 1. / z);
-        p_star += .5 * pr * pow(1. + (gamma - 1.) * (u_star - ur) / (2. * cr), 1. / z);
+        p_star = .5 * pl * pow_b1(1. + (gamma - 1.) * (ul - u_star) / (2. * cl), g.inv_z);
+        p_star += .5 * pr * pow_b1(1. + (gamma - 1.) * (u_star - ur) / (2. * cr), g.inv_z);
     } else {
         double Al = 2. / ((gamma + 1.) * dl);
         double Ar = 2. / ((gamma + 1.) * dr);
-        double Bl = pl * ((gamma - 1.) / (gamma + 1.));
-        double Br = pr * ((gamma - 1.) / (gamma + 1.));
+        double Bl = pl * g.gm1_gp1;
+        double Br = pr * g.gm1_gp1;
         double gl = sqrt(Al / (p_star + Bl));
         double gr = sqrt(Ar / (p_star + Br));
         p_star = (gl * pl + gr * pr - (ur - ul)) / (gl + gr);
     }
     if (p_star <= pl) sl = ul - cl;
-    else sl = ul - cl * sqrt(1. + ((gamma + 1.) / (2. * gamma)) * (p_star / pl - 1.));
+    else sl = ul - cl * sqrt(1. + g.gp1_2g * (p_star / pl - 1.));
     if (p_star <= pr) sr = ur + cr;
-    else sr = ur + cr * sqrt(1. + ((gamma + 1.) / (2. * gamma)) * (p_star / pr - 1.));
+    else sr = ur + cr * sqrt(1. + g.gp1_2g * (p_star / pr - 1.));
     sc = (pr - pl + dl * ul * (sl - ul) - dr * ur * (sr - ur)) / (dl * (sl - ul) - dr * (sr - ur));
 }
 
@@ -63,8 +97,9 @@ __device__ __forceinline__ void side_flux(const double* S, const double* n, doub
 // HLL (riemann.pyx:243-373) and HLLC (riemann.pyx:521-682) interface flux
 template <int D, int SOLVER>
 __device__ __forceinline__ void riemann_hll(const double* L, const double* R, const double* n,
-                                            const double* fv, double gamma, int boost, double* F)
+                                            const double* fv, const Gas& g, int boost, double* F)
 {
+    const double gamma = g.gamma;
     double _dl = L[0], _pl = L[D + 1], _dr = R[0], _pr = R[D + 1];
     double Vnl = 0.0, Vnr = 0.0, vl_sq = 0.0, vr_sq = 0.0, wn = 0.0;
 #pragma unroll
@@ -77,7 +112,7 @@ __device__ __forceinline__ void riemann_hll(const double* L, const double* R, co
     }
     if (boost) wn = 0.;
     double sl, sc, sr;
-    get_waves(_dl, Vnl, _pl, _dr, Vnr, _pr, gamma, sl, sc, sr);
+    get_waves(_dl, Vnl, _pl, _dr, Vnr, _pr, g, sl, sc, sr);
     if (wn <= sl) {
         side_flux<D>(L, n, Vnl, vl_sq, wn, gamma, F);
     } else if ((sl < wn) && (wn <= sr)) {
@@ -136,7 +171,7 @@ __device__ __forceinline__ void deboost(const double* fv, double* F)
 // ---- Exact solver (riemann.pyx:684-1090), Toro chapter 4 -------------------
 __device__ inline double ex_p_func(double d, double p, double c, double gamma, double p_old)
 {
-    if (p_old <= p) return 2. * c / (gamma - 1.) * (pow(p_old / p, (gamma - 1.) / (2. * gamma)) - 1.);
+    if (p_old <= p) return 2. * c / (gamma - 1.) * (pow_b1(p_old / p, (gamma - 1.) / (2. * gamma)) - 1.);
     double Ak = 2. / (d * (gamma + 1.));
     double Bk = p * (gamma - 1.) / (gamma + 1.);
     return (p_old - p) * sqrt(Ak / (p_old + Bk));
@@ -144,7 +179,7 @@ __device__ inline double ex_p_func(double d, double p, double c, double gamma, d
 
 __device__ inline double ex_p_deriv(double d, double p, double c, double gamma, double p_old)
 {
-    if (p_old <= p) return pow(p_old / p, -(gamma + 1.) / (2. * gamma)) / (c * d);
+    if (p_old <= p) return pow_b1(p_old / p, -(gamma + 1.) / (2. * gamma)) / (c * d);
     double Ak = 2. / (d * (gamma + 1.));
     double Bk = p * (gamma - 1.) / (gamma + 1.);
     return sqrt(Ak / (p_old + Bk)) * (1. - .5 * (p_old - p) / (Bk + p_old));
@@ -162,11 +197,11 @@ __device__ inline bool ex_pstar(double dl, double ul, double pl, double cl, doub
     if ((q_max <= 2.) && (p_min <= ppv && ppv <= p_max)) {
         p_old = ppv;
     } else if (ppv <= p_min) {
-        double p_lr = pow(pl / pr, (gamma - 1.) / (2. * gamma));
+        double p_lr = pow_b1(pl / pr, (gamma - 1.) / (2. * gamma));
         double u_star = (p_lr * ul / cl + ur / cr + 2. * (p_lr - 1.) / (gamma - 1.));
         u_star = u_star / (p_lr / cl + 1. / cr);
-        double p_tl = pow(1. + (gamma - 1.) * (ul - u_star) / (2. * cl), 2. * gamma / (gamma - 1.));
-        double p_tr = pow(1. + (gamma - 1.) * (u_star - ur) / (2. * cr), 2. * gamma / (gamma - 1.));
+        double p_tl = pow_b1(1. + (gamma - 1.) * (ul - u_star) / (2. * cl), 2. * gamma / (gamma - 1.));
+        double p_tr = pow_b1(1. + (gamma - 1.) * (u_star - ur) / (2. * cr), 2. * gamma / (gamma - 1.));
         p_old = .5 * (pl * p_tl + pr * p_tr);
     } else {
         double gl = sqrt((2. / (dl * (gamma + 1.))) / ((gamma - 1.) * pl / (gamma + 1.) + ppv));
@@ -224,12 +259,12 @@ __device__ bool riemann_exact(const double* L, const double* R, const double* n,
         if (p_star <= _pl) {
             if (0.0 <= vnl - cl) data = true;
             else {
-                double c_star_l = cl * pow(p_star / _pl, (gamma - 1.0) / (2.0 * gamma));
-                if (0.0 >= u_star - c_star_l) { d = _dl * pow(p_star / _pl, 1.0 / gamma); p = p_star; shift = u_star; }
+                double c_star_l = cl * pow_b1(p_star / _pl, (gamma - 1.0) / (2.0 * gamma));
+                if (0.0 >= u_star - c_star_l) { d = _dl * pow_b1(p_star / _pl, 1.0 / gamma); p = p_star; shift = u_star; }
                 else {
                     double c = (2.0 / (gamma + 1.0)) * (cl + 0.5 * (gamma - 1.0) * vnl);
-                    d = _dl * pow(c / cl, 2.0 / (gamma - 1.0));
-                    p = _pl * pow(c / cl, 2.0 * gamma / (gamma - 1.0));
+                    d = _dl * pow_b1(c / cl, 2.0 / (gamma - 1.0));
+                    p = _pl * pow_b1(c / cl, 2.0 * gamma / (gamma - 1.0));
                     shift = c;
                 }
             }
@@ -257,13 +292,13 @@ __device__ bool riemann_exact(const double* L, const double* R, const double* n,
         } else {
             if (0.0 >= vnr + cr) data = true;
             else {
-                double c_star_r = cr * pow(p_star / _pr, (gamma - 1.0) / (2.0 * gamma));
-                if (0.0 <= u_star + c_star_r) { d = _dr * pow(p_star / _pr, 1.0 / gamma); p = p_star; shift = u_star; }
+                double c_star_r = cr * pow_b1(p_star / _pr, (gamma - 1.0) / (2.0 * gamma));
+                if (0.0 <= u_star + c_star_r) { d = _dr * pow_b1(p_star / _pr, 1.0 / gamma); p = p_star; shift = u_star; }
                 else {
                     double c = (2.0 / (gamma + 1.0)) * (cr - 0.5 * (gamma - 1.0) * vnr);
                     double u_tmp = (2.0 / (gamma + 1.0)) * (-cr + 0.5 * (gamma - 1.0) * vnr);
-                    d = _dr * pow(c / cr, 2.0 / (gamma - 1.0));
-                    p = _pr * pow(c / cr, 2.0 * gamma / (gamma - 1.0));
+                    d = _dr * pow_b1(c / cr, 2.0 / (gamma - 1.0));
+                    p = _pr * pow_b1(c / cr, 2.0 * gamma / (gamma - 1.0));
                     shift = u_tmp;
                 }
             }
@@ -290,14 +325,14 @@ __device__ bool riemann_exact(const double* L, const double* R, const double* n,
 // Exact always deboosts (972).
 template <int D, int SOLVER>
 __device__ __forceinline__ bool solve_face(const double* L, const double* R, const double* n,
-                                           const double* fv, double gamma, int boost, double* F)
+                                           const double* fv, const Gas& g, int boost, double* F)
 {
     if (SOLVER == PHD_SOLVER_EXACT) {
-        if (!riemann_exact<D>(L, R, n, gamma, F)) return false;
+        if (!riemann_exact<D>(L, R, n, g.gamma, F)) return false;
         deboost<D>(fv, F);
         return true;
     }
-    riemann_hll<D, SOLVER>(L, R, n, fv, gamma, boost, F);
+    riemann_hll<D, SOLVER>(L, R, n, fv, g, boost, F);
     if (boost) deboost<D>(fv, F);
     return true;
 }

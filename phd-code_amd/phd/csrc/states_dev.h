@@ -18,38 +18,20 @@ __device__ __forceinline__ void boost_copy(const double* pi, const double* pj, c
     }
 }
 
-// PieceWiseLinear.add_spatial body (reconstruction.pyx:849-921).
-//   com   : face centroid;  ci, cj : cell centroids x + dcom (861-862)
-//   diff  : |x_j - x_i| (866, 875)
-//   pi,pj : lab-frame primitives (the GIZMO branch test at 902/906)
+// GIZMO pairwise limiter, Hopkins (2015) appendix B4 (reconstruction.pyx:
+// 877-921), applied to the extrapolated states L, R.
+//   sepi_mag, sepj_mag : SQUARED distances face centroid - cell centroid
+//                        (the square roots are taken here, as in the reference)
+//   sl, sr             : the face-frame cell states before extrapolation
 template <int D>
-__device__ __forceinline__ void spatial_states(const double* com, double diff, const double* ci,
-                                               const double* cj, const double* pi,
-                                               const double* pj, const double* gi,
-                                               const double* gj, int gizmo, double* L, double* R)
+__device__ __forceinline__ void gizmo_limit(double sepi_mag, double sepj_mag, double diff,
+                                            const double* pi, const double* pj, const double* sl,
+                                            const double* sr, double* L, double* R)
 {
     constexpr int NF = D + 2;
     const double psi1 = 0.5, psi2 = 0.25;
-    double sl[NF], sr[NF];
-#pragma unroll
-    for (int n = 0; n < NF; n++) { sl[n] = L[n]; sr[n] = R[n]; }
-    double sepi_mag = 0.0, sepj_mag = 0.0;
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-        double sepi = com[k] - ci[k];
-        double sepj = com[k] - cj[k];
-        sepi_mag += sepi * sepi;
-        sepj_mag += sepj * sepj;
-#pragma unroll
-        for (int n = 0; n < NF; n++) {
-            L[n] += gi[n * D + k] * sepi;
-            R[n] += gj[n * D + k] * sepj;
-        }
-    }
-    if (!gizmo) return;
     sepi_mag = sqrt(sepi_mag);
     sepj_mag = sqrt(sepj_mag);
-    // GIZMO pairwise limiter, Hopkins (2015) appendix B4 (877-912)
 #pragma unroll
     for (int n = 0; n < NF; n++) {
         double delta1 = psi1 * fabs(sl[n] - sr[n]);
@@ -83,6 +65,36 @@ __device__ __forceinline__ void spatial_states(const double* com, double diff, c
 #pragma unroll
         for (int n = 0; n < NF; n++) R[n] = sr[n];
     }
+}
+
+// PieceWiseLinear.add_spatial body (reconstruction.pyx:849-921).
+//   com   : face centroid;  ci, cj : cell centroids x + dcom (861-862)
+//   diff  : |x_j - x_i| (866, 875)
+//   pi,pj : lab-frame primitives (the GIZMO branch test at 902/906)
+template <int D>
+__device__ __forceinline__ void spatial_states(const double* com, double diff, const double* ci,
+                                               const double* cj, const double* pi,
+                                               const double* pj, const double* gi,
+                                               const double* gj, int gizmo, double* L, double* R)
+{
+    constexpr int NF = D + 2;
+    double sl[NF], sr[NF];
+#pragma unroll
+    for (int n = 0; n < NF; n++) { sl[n] = L[n]; sr[n] = R[n]; }
+    double sepi_mag = 0.0, sepj_mag = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        double sepi = com[k] - ci[k];
+        double sepj = com[k] - cj[k];
+        sepi_mag += sepi * sepi;
+        sepj_mag += sepj * sepj;
+#pragma unroll
+        for (int n = 0; n < NF; n++) {
+            L[n] += gi[n * D + k] * sepi;
+            R[n] += gj[n * D + k] * sepj;
+        }
+    }
+    if (gizmo) gizmo_limit<D>(sepi_mag, sepj_mag, diff, pi, pj, sl, sr, L, R);
 }
 
 // one side of PieceWiseLinear.add_temporal (reconstruction.pyx:1018-1072):

@@ -96,6 +96,52 @@ def test_arepo_limiter_bounds_and_maximality(dim, n):
                     assert gap.min() < 1e-10
 
 
+@pytest.mark.parametrize("dim,n", [(2, 30), (3, 10)])
+def test_tess_limiter_scales_density_only(dim, n):
+    """reconstruction.pyx:705-736: one alpha, min over every field and face
+    of max((phi_j - phi_i)/dphi, 0), applied to the density gradient only.
+    Density is given the pressure's values, so its unlimited gradient is the
+    (unlimited) pressure gradient bit for bit."""
+    a = lattice(n, dim, seed=6, boundary=1)
+    rng = np.random.default_rng(2)
+    nf = dim + 2
+    a["prim"] = [1.0 + 0.3 * rng.random(a["n"]) for _ in range(nf)]
+    a["prim"][0] = a["prim"][nf - 1].copy()
+    a["prim"] = ghost_copy(a, a["prim"], flip_velocity=False)
+    tess, rc, _ = oracle.compute_gradients(a, limiter=1)
+    assert rc == 0
+    nr = a["n_real"]
+    off, fl = a["nbr_off"], a["nbr_face"]
+    pi, pj = a["pair_i"], a["pair_j"]
+    touched = 0
+    for i in range(0, nr, 5):
+        g0 = np.array([tess[(nf - 1) * dim + k][i] for k in range(dim)])
+        gd = np.array([tess[k][i] for k in range(dim)])
+        if np.abs(g0).max() < 1e-300:
+            continue
+        alpha = gd[np.argmax(np.abs(g0))] / g0[np.argmax(np.abs(g0))]
+        assert -1e-15 <= alpha <= 1.0 + 1e-15
+        np.testing.assert_allclose(gd, alpha * g0, rtol=1e-13, atol=1e-15)
+        faces = fl[off[i]:off[i + 1]]
+        nb = np.where(pi[faces] == i, pj[faces], pi[faces])
+        cx = np.array([a["x"][k][i] + a["dcom"][k][i] for k in range(dim)])
+        rel = np.stack([a["com"][k][faces] for k in range(dim)], 1) - cx
+        tight = False
+        for f in range(nf):
+            gf = np.array([tess[f * dim + k][i] for k in range(dim)]) if f else g0
+            dphi = rel @ gf
+            jump = a["prim"][f][nb] - a["prim"][f][i]
+            nz = dphi != 0.0
+            psi = np.maximum(jump[nz] / dphi[nz], 0.0)
+            # alpha never exceeds any face's ratio, for any field ...
+            assert np.all(alpha <= psi + 1e-12)
+            # ... and is attained by one of them (or is 1)
+            tight |= bool(np.any(np.abs(psi - alpha) < 1e-12))
+        assert tight or abs(alpha - 1.0) < 1e-15
+        touched += alpha < 1.0
+    assert touched > 0
+
+
 def test_ghost_gradient_flip_rule():
     """boundary.pyx:291-307: only d v_q/d x_q flips, and only for exterior
     ghosts outside the box on axis q; periodic ghosts are pure copies."""
