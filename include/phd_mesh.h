@@ -47,7 +47,8 @@ phdm_mesh* phdm_build(int dim, int64_t n_real, const double* const* x,
 int phdm_counts(const phdm_mesh* m, int64_t* n_ghost, int64_t* n_faces,
                 int64_t* n_list);
 /* Copy results into caller arrays (host memory):
- *   ghost_x[k][g], ghost_map[g], ghost_type[g]   (g < n_ghost)
+ *   ghost_x[k][g], ghost_map[g]                  (g < n_ghost; every ghost is
+ *                                                an exterior image)
  *   volume[c], dcom[k][c]                        (c < n_real + n_ghost)
  *   area[f], com[k][f], normal[k][f], pair_i[f], pair_j[f]
  *   nbr_off[c] (n_real + n_ghost + 1), nbr_face[n_list]                  */

@@ -846,7 +846,7 @@ int orc_update_from_fluxes(int dim, int nf, double dt, const int8_t* tag,
 /* Core face-flux step on a frozen mesh, in the order of
  * MovingMeshMUSCLHancock.evolve_timestep (integrate.py:396-408):
  * gradients + limiter, ghost gradients, spatial + temporal states at dt/2,
- * HLLC (or HLL) + deboost, conserved update.  Used as the timed CPU
+ * HLL, HLLC or Exact + deboost, conserved update.  Used as the timed CPU
  * baseline; states and fluxes live in caller-provided scratch. */
 int orc_core_step(int dim, int n, int nf, double gamma, double dt, int solver,
                   int boost, int gizmo, int limiter, int boundary,
@@ -869,8 +869,14 @@ int orc_core_step(int dim, int n, int nf, double gamma, double dt, int solver,
                     gizmo, pl, pr);
     orc_add_temporal(dim, nf, gamma, 0.5*dt, prim, grad, fv, pair_i, pair_j,
                      boost, pl, pr);
+    /* riemann.compute_fluxes (riemann.pyx:74-98): HLL 0, HLLC 1, Exact 2;
+     * the Exact solver raises before the update when Newton fails */
     if (solver == 0) orc_hll(dim, nf, gamma, boost, pl, pr, nx, fv, flux);
-    else orc_hllc(dim, nf, gamma, boost, pl, pr, nx, fv, flux);
+    else if (solver == 1) orc_hllc(dim, nf, gamma, boost, pl, pr, nx, fv, flux);
+    else {
+        rc = orc_exact(dim, nf, gamma, pl, pr, nx, fv, flux, bad);
+        if (rc) return rc;
+    }
     return orc_update_from_fluxes(dim, nf, dt, tag, area, pair_i, pair_j, flux,
                                   cons, bad);
 }
