@@ -151,15 +151,19 @@ def cpu_baseline(problem, dt):
     from oracle import oracle
     a = problems.host_arrays(problem)
     oracle.lib()
-    t0 = time.perf_counter()
-    r = oracle.core_step(a, dt)
-    t = time.perf_counter() - t0
-    if r["rc"] != 0:
-        raise RuntimeError("oracle core step reported fault %d" % r["rc"])
+    times = []
+    while len(times) < 3 and sum(times) < 20.0:
+        t0 = time.perf_counter()
+        r = oracle.core_step(a, dt)
+        times.append(time.perf_counter() - t0)
+        if r["rc"] != 0:
+            raise RuntimeError("oracle core step reported fault %d" % r["rc"])
+    t = sorted(times)[len(times) // 2]
     return {"value": a["nf"] / t, "unit": "face-updates/s", "cores": 1, "kind": "port",
-            "sample": "one full core step (gradients+limiter, ghost gradients, spatial+temporal "
-                      "states, HLLC+deboost, update) of the same 128^3 Sedov mesh, %d faces, "
-                      "%.2f s on one core" % (a["nf"], t)}
+            "sample": "median of %d full core steps (gradients+limiter, ghost gradients, "
+                      "spatial+temporal states, HLLC+deboost, update) of the same Sedov mesh, "
+                      "%d faces: %s s each on one core" %
+                      (len(times), a["nf"], ", ".join("%.2f" % x for x in times))}
 
 
 def main():

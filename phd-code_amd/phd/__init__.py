@@ -13,6 +13,7 @@ from .domain import DomainManager, BoundaryConditionBase, Reflective, Periodic
 from .reconstruction import ReconstructionBase, PieceWiseConstant, PieceWiseLinear
 from .riemann import RiemannBase, HLL, HLLC, Exact
 from .integrate import IntegrateBase, StaticMeshMUSCLHancock, MovingMeshMUSCLHancock
+from .read_write import ReaderWriterBase, Hdf5, NpzTree
 from ._native import NativeUnavailable
 
 # single GPU: no MPI communicator (the reference's phd._comm globals)

@@ -118,3 +118,34 @@ def test_compute_without_gpu_raises():
     pc.register_carray(4, "volume", "double")
     with pytest.raises(phd.NativeUnavailable):
         phd.IdealGas().conservative_from_primitive(pc)
+
+
+def test_snapshot_tree_round_trip(tmp_path):
+    """NpzTree writes the reference's Hdf5 tree (io/read_write.py:53-80) and
+    reads back the same fields, dtypes and values."""
+    pc = phd.HydroParticleCreator(num=6, dim=2, device="cpu")
+    pc["density"][:] = torch.linspace(1.0, 2.0, 6, dtype=torch.float64)
+    pc["tag"][:] = torch.tensor([0, 0, 0, 0, 1, 1], dtype=torch.int8)
+    pc["ids"][:] = torch.arange(6, dtype=torch.int32)
+
+    class Integ(object):
+        dt, time, iteration, particles = 0.01, 0.5, 7, pc
+
+    path = phd.NpzTree().write("snap_0007", str(tmp_path), Integ())
+    z = np.load(path, allow_pickle=False)
+    assert float(z["attrs/time"]) == 0.5 and int(z["attrs/iteration"]) == 7
+    assert int(z["particles/attrs/number_particles"]) == 6
+    assert int(z["particles/attrs/Ghost"]) == phd.ParticleTAGS.Ghost
+    assert str(z["particles/tag/attrs/dtype"]) == "int"
+    back = phd.NpzTree().read(path, device="cpu")
+    assert set(back.carrays) == set(pc.carrays)
+    for name in pc.carrays:
+        assert back.carray_dtypes[name] == pc.carray_dtypes[name]
+        assert torch.equal(back[name], pc[name])
+    with pytest.raises(RuntimeError, match="h5py"):
+        try:
+            import h5py  # noqa: F401
+        except ImportError:
+            phd.Hdf5().write("snap", str(tmp_path), Integ())
+        else:
+            raise RuntimeError("h5py present: the Hdf5 writer is exercised instead")
