@@ -11,7 +11,14 @@
 //                     cell's ascending face list (the order of
 //                     Tess2d/3d::extract_geometry, tess.cpp:379-382)
 //     xp[Na][8]       position(D) | primitive(D+2), packed per step  64 B/cell
-//     g[F][D+2]       (dt*area)*flux rows written by the face kernel
+//     perm[Na]        visiting order of the sliced lists (degree-sorted
+//                     windows of 256 cells)
+//     sliced lists    per slot: other cell, face id | incoming bit, face
+//                     centroid, area
+//     g[F][GW]        (dt*area)*flux per face, one row of D+2 doubles padded
+//                     to an even count, written by the face kernel
+//                     (coalesced) and applied by the update as -g to the
+//                     owner and +g to the neighbour
 //   Cell record rows rec[Na][ld] (the gradient container): primitive(D+2) |
 //   gradient(D(D+2)) | centroid(D); written by the gradient kernels, read
 //   whole by the face kernel (one 192 B row per 3D cell).
@@ -54,18 +61,23 @@ struct phd_plan {
     int2* csr;
     int32_t* off;
     double* xp;
-    double* g;
     unsigned long long* d_bad;
     long long bad_cell;          // first real cell with an inconsistent face list, or -1
-    // sliced lists (16 cells per slice): entry r of cell c lives in slot
-    // sbase[c/16] + 16 r + c%16, so a wavefront reads 4 x 16 consecutive slots
+    // sliced lists: cells are visited in the order perm (within each window
+    // of 256 consecutive cells, by descending face count; reals before
+    // ghosts), and the cell at position t keeps entry r of its face list in
+    // slot sbase[t/16] + 16 r + t%16.  A wavefront reads 4 x 16 consecutive
+    // slots per entry, and its 64 lanes have nearly equal list lengths.
+    int32_t* perm;               // position -> cell, n_cells
     long long n_slots;
     void* slot_base;             // second allocation holding the slot arrays
     int32_t* sbase;              // n_slices + 1
     int32_t* sj;                 // other cell | incoming bit (INVALID for padding)
+    int32_t* sfi;                // face id | incoming bit (INVALID for padding)
     double* sgeo[4];             // face centroid x, y, z, area per slot
-    double* sg[5];               // per-slot update summands (owner side negated)
-    int32_t* fslot;              // per face: owner slot, neighbour slot (-1 for ghosts)
+    double* g;                   // per-face update summand rows (dt*area)*flux, F x GW
+    int owner_runs;              // every real cell's owned faces are one id run that
+                                 // follows all its incoming faces (faces in owner order)
 };
 
 using namespace phd;
@@ -81,6 +93,9 @@ constexpr int GEO = 4;       // doubles per cell geometry row
 constexpr int XPW = 8;       // doubles per packed cell row
 
 struct FaceArrays { double* q[5]; };
+
+// doubles per summand row: D+2 padded to even (16-byte aligned rows)
+template <int D> constexpr int summand_width() { return (D + 3) & ~1; }
 
 template <int D> struct Lay {
     static constexpr int NF = D + 2;                 // primitive fields
@@ -315,18 +330,68 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_plan_csr(phd_cells c, phd_faces f
     }
 }
 
+// owner_runs check: in every real cell's list, the owned faces (not
+// incoming) are consecutive ids after all incoming faces; clears flag[0]
+// (NOERR) otherwise
+__global__ __launch_bounds__(PHD_BLOCK) void k_plan_runs(phd_cells c, const int2* csr,
+                                                         const int32_t* off,
+                                                         unsigned long long* flag)
+{
+    long long i = xcd_index(c.n_cells);
+    if (i >= c.n_cells || c.tag[i] != 0) return;
+    bool owned = false, ok = true;
+    int prev = 0;
+    for (int e = off[i]; e < off[i + 1]; e++) {
+        const int2 en = csr[e];
+        if ((en.y & 0x7fffffff) == INVALID) { ok = false; break; }
+        if (en.y & INCOMING) {
+            if (owned) { ok = false; break; }
+        } else {
+            if (owned && en.x != prev + 1) { ok = false; break; }
+            owned = true;
+            prev = en.x;
+        }
+    }
+    if (!ok) atomicMin(flag, 0ull);
+}
+
 constexpr int SLICE = 16;
+constexpr int SORT_WINDOW = PHD_BLOCK;   // cells per degree-sorted window
 
 // width of each 16-cell slice = the largest face-list length of its real cells
+// position -> cell order: inside each window of SORT_WINDOW consecutive
+// cells, real cells by descending face count (ties by id), then ghosts
+__global__ __launch_bounds__(PHD_BLOCK) void k_plan_perm(phd_cells c, const int32_t* off,
+                                                         int32_t* perm)
+{
+    __shared__ int key[SORT_WINDOW];
+    const long long w0 = (long long)blockIdx.x * SORT_WINDOW;
+    const int l = threadIdx.x;
+    const long long i = w0 + l;
+    int k = -2;                                   // past the end
+    if (i < c.n_cells) k = c.tag[i] == 0 ? off[i + 1] - off[i] : -1;
+    key[l] = k;
+    __syncthreads();
+    int rank = 0;
+    for (int m = 0; m < SORT_WINDOW; m++) {
+        const int km = key[m];
+        rank += (km > k) | ((km == k) & (m < l));
+    }
+    if (i < c.n_cells) perm[w0 + rank] = (int32_t)i;
+}
+
 __global__ __launch_bounds__(PHD_BLOCK) void k_plan_slice_width(phd_cells c, const int32_t* off,
+                                                                const int32_t* perm,
                                                                 int32_t* width, int n_slices)
 {
     long long s = (long long)blockIdx.x * PHD_BLOCK + threadIdx.x;
     if (s >= n_slices) return;
     int w = 0;
     for (int l = 0; l < SLICE; l++) {
-        long long i = s * SLICE + l;
-        if (i < c.n_cells && c.tag[i] == 0) w = max(w, off[i + 1] - off[i]);
+        long long t = s * SLICE + l;
+        if (t >= c.n_cells) break;
+        const int i = perm[t];
+        if (c.tag[i] == 0) w = max(w, off[i + 1] - off[i]);
     }
     width[s] = w;
 }
@@ -341,16 +406,17 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_fill_i32(int32_t* a, long long n,
 template <int D>
 __global__ __launch_bounds__(PHD_BLOCK) void k_plan_slots(phd_cells c, phd_faces f,
                                                           const int2* csr, const int32_t* off,
+                                                          const int32_t* perm,
                                                           const int32_t* sbase, int32_t* sj,
                                                           double* sx, double* sy, double* sz,
-                                                          double* sa, int32_t* fslot)
+                                                          double* sa, int32_t* sfi)
 {
-    long long ii = xcd_index(c.n_cells);
-    if (ii >= c.n_cells) return;
-    const int i = (int)ii;
+    long long t = xcd_index(c.n_cells);
+    if (t >= c.n_cells) return;
+    const int i = perm[t];
     if (c.tag[i] != 0) return;
     const int beg = off[i], end = off[i + 1];
-    const long long base = sbase[i / SLICE] + (i % SLICE);
+    const long long base = sbase[t / SLICE] + (t % SLICE);
     for (int e = beg; e < end; e++) {
         const int2 en = csr[e];
         const long long slot = base + (long long)(e - beg) * SLICE;
@@ -360,9 +426,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_plan_slots(phd_cells c, phd_faces
         sy[slot] = f.com[1][fid];
         sz[slot] = D == 3 ? f.com[D - 1][fid] : 0.0;
         sa[slot] = f.area[fid];
-        if ((en.y & 0x7fffffff) == INVALID) continue;
-        if (en.y & INCOMING) fslot[2 * (size_t)fid + 1] = (int32_t)slot;
-        else fslot[2 * (size_t)fid] = (int32_t)slot;
+        sfi[slot] = (en.y & 0x7fffffff) == INVALID ? INVALID : (fid | (en.y & INCOMING));
     }
 }
 
@@ -466,6 +530,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_grad_sell(int n_cells, const int8
                                                          const double* __restrict__ geo,
                                                          const double* __restrict__ xp,
                                                          const int32_t* __restrict__ off,
+                                                         const int32_t* __restrict__ perm,
                                                          const int32_t* __restrict__ sbase,
                                                          const int32_t* __restrict__ sj,
                                                          const double* __restrict__ sx,
@@ -476,9 +541,9 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_grad_sell(int n_cells, const int8
 {
     using Y = Lay<D>;
     constexpr int NF = Y::NF;
-    long long ii = xcd_index(n_cells);
-    if (ii >= n_cells) return;
-    const int i = (int)ii;
+    const long long t = xcd_index(n_cells);
+    if (t >= n_cells) return;
+    const int i = perm[t];
     if (tag[i] != 0) return;
 
     double cx[D], vol, xi[D], pv[NF];
@@ -505,7 +570,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_grad_sell(int n_cells, const int8
         for (int k = 0; k < D; k++) df[n * D + k] = 0.0;
     }
     const int deg = off[i + 1] - off[i];
-    const long long base = sbase[i / SLICE] + (i % SLICE);
+    const long long base = sbase[t / SLICE] + (t % SLICE);
     for (int e = 0; e < deg; e++) {
         const SlotEnt<D> cur = load_slot<D>(sj, sx, sy, sz, sa, base + (long long)e * SLICE);
         const int j = cur.j;
@@ -761,15 +826,15 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_riemann(phd_faces f, FaceArrays l
 
 // Fused states + Riemann + deboost; the states never leave registers.
 // OUT bit 1: write the flux arrays (riemann.fluxes); bit 2: write
-// g[m] = (dt_full*area)*flux, the update's summands (mesh.pyx:530-544).
+// g[m] = (dt_full*area)*flux, the update's summand (mesh.pyx:530-544 forms
+// dt*a*flux for each endpoint; the product is the same for both).
 template <int D, int ORDER, int SOLVER, int OUT>
 __global__ __launch_bounds__(PHD_BLOCK) void k_face(phd_cells c, phd_faces f,
                                                     const double* __restrict__ frec,
                                                     const double* __restrict__ rec, int ld,
                                                     Gas gas, double dt_half, double dt_full,
                                                     int boost, int gizmo, FaceArrays flux,
-                                                    FaceArrays sg,
-                                                    const int32_t* __restrict__ fslot,
+                                                    double* __restrict__ g,
                                                     unsigned long long* err)
 {
     constexpr int NF = D + 2;
@@ -791,16 +856,14 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_face(phd_cells c, phd_faces f,
         for (int q = 0; q < NF; q++) flux.q[q][m] = F[q];
     }
     if (OUT & 2) {
-        // summands for the two endpoints' sliced lists: (dt*area)*F enters
-        // the neighbour as +G and the owner as -G (u - G == u + (-G) exactly)
+        constexpr int GW = summand_width<D>();
         const double t = dt_full * area;
-        const int2 fs = *reinterpret_cast<const int2*>(fslot + 2 * (size_t)m);
+        double row[GW];
 #pragma unroll
-        for (int q = 0; q < NF; q++) {
-            const double gq = t * F[q];
-            if (fs.x >= 0) sg.q[q][fs.x] = -gq;
-            if (fs.y >= 0) sg.q[q][fs.y] = gq;
-        }
+        for (int q = 0; q < GW; q++) row[q] = q < NF ? t * F[q] : 0.0;
+        double2* o = reinterpret_cast<double2*>(g + (size_t)m * GW);
+#pragma unroll
+        for (int q = 0; q < GW / 2; q++) o[q] = make_double2(row[2 * q], row[2 * q + 1]);
     }
 }
 
@@ -808,15 +871,48 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_face(phd_cells c, phd_faces f,
 // conserved update (mesh.pyx:479-551).  One thread per real cell walks the
 // cell's face list in ascending face id, the order in which the reference's
 // face loop reaches that cell, so every running sum is bit-identical and no
-// atomics are needed.  The positivity check (546-551) fires at the first
-// face n where a running mass or energy of either endpoint is <= 0; each
-// endpoint reports its first failure as key 2n (mass) or 2n+1 (energy) and
-// the minimum key wins.  Ghost endpoints never change: a ghost fails at the
-// first face of its list.  SRC 0 reads flux arrays + area, SRC 1 the g rows.
-template <int D, int SRC>
+// atomics are needed: the owner (pair_i) subtracts (dt*area)*flux, the
+// neighbour adds it.  The positivity check (546-551) fires at the first face
+// n where a running mass or energy of either endpoint is <= 0; each endpoint
+// reports its first failure as key 2n (mass) or 2n+1 (energy) and the
+// minimum key wins.  Ghost endpoints never change: a ghost fails at the
+// first face of its list.
+template <int D>
+__device__ __forceinline__ bool ghost_fault(const phd_cells& c, int i, int beg, int end,
+                                            const int2* csr, unsigned long long* err)
+{
+    if (c.tag[i] == 0) return false;
+    if (beg < end) {
+        const bool bm = c.cons[0][i] <= 0.0, be = c.cons[D + 1][i] <= 0.0;
+        if (bm || be) atomicMin(&err[0], 2ull * (unsigned long long)csr[beg].x + (bm ? 0ull : 1ull));
+    }
+    return true;
+}
+
+template <int D>
+__device__ __forceinline__ bool apply_summand(double* u, const double* t, bool incoming,
+                                              int fid, unsigned long long* err)
+{
+    constexpr int NF = D + 2;
+    if (incoming) {
+#pragma unroll
+        for (int q = 0; q < NF; q++) u[q] += t[q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < NF; q++) u[q] -= t[q];
+    }
+    const bool bm = u[0] <= 0.0, be = u[D + 1] <= 0.0;
+    if (bm || be) {
+        atomicMin(&err[0], 2ull * (unsigned long long)fid + (bm ? 0ull : 1ull));
+        return true;
+    }
+    return false;
+}
+
+// API path: riemann.fluxes + face areas, per-cell lists from the CSR
+template <int D>
 __global__ __launch_bounds__(PHD_BLOCK) void k_update(phd_cells c, const double* __restrict__ area,
                                                       FaceArrays flux,
-                                                      const double* __restrict__ g,
                                                       const int2* __restrict__ csr,
                                                       const int32_t* __restrict__ off, double dt,
                                                       unsigned long long* err)
@@ -825,16 +921,8 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_update(phd_cells c, const double*
     long long ii = xcd_index(c.n_cells);
     if (ii >= c.n_cells) return;
     const int i = (int)ii;
-    const int8_t tg = c.tag[i];
     const int beg = off[i], end = off[i + 1];
-    if (tg != 0) {
-        if (beg < end) {
-            const bool bm = c.cons[0][i] <= 0.0, be = c.cons[D + 1][i] <= 0.0;
-            if (bm || be)
-                atomicMin(&err[0], 2ull * (unsigned long long)csr[beg].x + (bm ? 0ull : 1ull));
-        }
-        return;
-    }
+    if (ghost_fault<D>(c, i, beg, end, csr, err)) return;
     double u[NF];
 #pragma unroll
     for (int q = 0; q < NF; q++) u[q] = c.cons[q][i];
@@ -843,72 +931,71 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_update(phd_cells c, const double*
         if ((en.y & 0x7fffffff) == INVALID) continue;
         const int fid = en.x;
         double t[NF];
-        if (SRC == 1) {
-            const double* row = g + (size_t)fid * NF;
+        const double s = dt * area[fid];
 #pragma unroll
-            for (int q = 0; q < NF; q++) t[q] = row[q];
-        } else {
-            const double s = dt * area[fid];
-#pragma unroll
-            for (int q = 0; q < NF; q++) t[q] = s * flux.q[q][fid];
-        }
-        if (en.y & INCOMING) {
-#pragma unroll
-            for (int q = 0; q < NF; q++) u[q] += t[q];
-        } else {
-#pragma unroll
-            for (int q = 0; q < NF; q++) u[q] -= t[q];
-        }
-        const bool bm = u[0] <= 0.0, be = u[D + 1] <= 0.0;
-        if (bm || be) {
-            atomicMin(&err[0], 2ull * (unsigned long long)fid + (bm ? 0ull : 1ull));
-            break;
-        }
+        for (int q = 0; q < NF; q++) t[q] = s * flux.q[q][fid];
+        if (apply_summand<D>(u, t, en.y & INCOMING, fid, err)) break;
     }
 #pragma unroll
     for (int q = 0; q < NF; q++) c.cons[q][i] = u[q];
 }
 
-// conserved update over the sliced lists: the slots of cell i hold its face
-// summands in ascending face id (owner side already negated), so the walk
-// u += s_0, u += s_1, ... is the reference's running sum, bit for bit.
+// fused-step path: the per-face summand rows g of the face kernel, the
+// per-cell lists as sliced face ids (sfi, coalesced) in the plan's visiting
+// order.  Faces are numbered in owner order, so a cell's list is its
+// incoming faces followed by one run of consecutive owned face ids
+// (owner_runs): the incoming rows are gathered, the owned run is read as
+// consecutive rows without touching the list again.
 template <int D>
-__global__ __launch_bounds__(PHD_BLOCK) void k_update_sell(phd_cells c, FaceArrays sg,
+__device__ __forceinline__ void load_summand(const double* g, int fid, double* s)
+{
+    constexpr int GW = summand_width<D>();
+    const double2* r = reinterpret_cast<const double2*>(g + (size_t)fid * GW);
+#pragma unroll
+    for (int q = 0; q < GW / 2; q++) {
+        const double2 v = r[q];
+        s[2 * q] = v.x;
+        s[2 * q + 1] = v.y;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(PHD_BLOCK) void k_update_sell(phd_cells c,
+                                                           const double* __restrict__ g,
+                                                           int owner_runs,
                                                            const int2* __restrict__ csr,
                                                            const int32_t* __restrict__ off,
+                                                           const int32_t* __restrict__ perm,
                                                            const int32_t* __restrict__ sbase,
-                                                           const int32_t* __restrict__ sj,
+                                                           const int32_t* __restrict__ sfi,
                                                            unsigned long long* err)
 {
     constexpr int NF = D + 2;
-    long long ii = xcd_index(c.n_cells);
-    if (ii >= c.n_cells) return;
-    const int i = (int)ii;
-    const int8_t tg = c.tag[i];
+    const long long t = xcd_index(c.n_cells);
+    if (t >= c.n_cells) return;
+    const int i = perm[t];
     const int beg = off[i], end = off[i + 1];
-    if (tg != 0) {
-        if (beg < end) {
-            const bool bm = c.cons[0][i] <= 0.0, be = c.cons[D + 1][i] <= 0.0;
-            if (bm || be)
-                atomicMin(&err[0], 2ull * (unsigned long long)csr[beg].x + (bm ? 0ull : 1ull));
-        }
-        return;
-    }
+    if (ghost_fault<D>(c, i, beg, end, csr, err)) return;
     double u[NF];
 #pragma unroll
     for (int q = 0; q < NF; q++) u[q] = c.cons[q][i];
     const int deg = end - beg;
-    const long long base = sbase[i / SLICE] + (i % SLICE);
+    const long long base = sbase[t / SLICE] + (t % SLICE);
     for (int e = 0; e < deg; e++) {
-        const long long slot = base + (long long)e * SLICE;
-        if ((__builtin_nontemporal_load(sj + slot) & 0x7fffffff) == INVALID) continue;
-#pragma unroll
-        for (int q = 0; q < NF; q++) u[q] += __builtin_nontemporal_load(sg.q[q] + slot);
-        const bool bm = u[0] <= 0.0, be = u[D + 1] <= 0.0;
-        if (bm || be) {
-            atomicMin(&err[0], 2ull * (unsigned long long)csr[beg + e].x + (bm ? 0ull : 1ull));
+        const int en = __builtin_nontemporal_load(sfi + base + (long long)e * SLICE);
+        if (en == INVALID) continue;
+        const int fid = en & 0x7fffffff;
+        double s[summand_width<D>()];
+        if (owner_runs && !(en & INCOMING)) {
+            // the rest of the list: owned faces fid, fid+1, ..
+            for (int k = 0; k < deg - e; k++) {
+                load_summand<D>(g, fid + k, s);
+                if (apply_summand<D>(u, s, false, fid + k, err)) break;
+            }
             break;
         }
+        load_summand<D>(g, fid, s);
+        if (apply_summand<D>(u, s, en & INCOMING, fid, err)) break;
     }
 #pragma unroll
     for (int q = 0; q < NF; q++) c.cons[q][i] = u[q];
@@ -1027,12 +1114,7 @@ void bounds3(int dim, int boundary, const double* lo, const double* hi, double* 
         for (int k = 0; k < dim; k++) { l[k] = lo[k]; u[k] = hi[k]; }
 }
 
-FaceArrays slot_g(const phd_plan* p)
-{
-    FaceArrays a;
-    for (int q = 0; q < 5; q++) a.q[q] = p->sg[q];
-    return a;
-}
+
 
 template <int D>
 int launch_pack(phd_handle* h, phd_plan* p, const phd_cells* c)
@@ -1052,7 +1134,7 @@ int launch_grad(phd_handle* h, phd_plan* p, const phd_cells* c, int limiter, dou
         LAUNCHCHK();
     }
     const unsigned nb = xcd_grid(c->n_cells);
-#define GS(L) k_grad_sell<D, L><<<nb, PHD_BLOCK, 0, h->stream>>>(c->n_cells, c->tag, p->cell_geo, p->xp, p->off, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], rec, ld)
+#define GS(L) k_grad_sell<D, L><<<nb, PHD_BLOCK, 0, h->stream>>>(c->n_cells, c->tag, p->cell_geo, p->xp, p->off, p->perm, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], rec, ld)
     if (limiter == PHD_LIMITER_AREPO) GS(PHD_LIMITER_AREPO);
     else if (limiter == PHD_LIMITER_TESS) GS(PHD_LIMITER_TESS);
     else return PHD_E_MODE;
@@ -1085,7 +1167,7 @@ int launch_face_o(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_face
 {
     const unsigned nb = xcd_grid(f->n_faces);
     switch (solver) {
-#define FACE(S) k_face<D, ORDER, S, OUT><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, p->face_rec, rec, ld, make_gas(gamma), dt_half, dt_full, boost, gizmo, flux, slot_g(p), p->fslot, h->d_err)
+#define FACE(S) k_face<D, ORDER, S, OUT><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, p->face_rec, rec, ld, make_gas(gamma), dt_half, dt_full, boost, gizmo, flux, p->g, h->d_err)
     case PHD_SOLVER_HLL: FACE(PHD_SOLVER_HLL); break;
     case PHD_SOLVER_HLLC: FACE(PHD_SOLVER_HLLC); break;
     case PHD_SOLVER_EXACT: FACE(PHD_SOLVER_EXACT); break;
@@ -1116,10 +1198,10 @@ int launch_update(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_face
     if (c->n_cells == 0) return 0;
     if (SRC == 1)
         k_update_sell<D><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(
-            *c, slot_g(p), p->csr, p->off, p->sbase, p->sj, h->d_err);
+            *c, p->g, p->owner_runs, p->csr, p->off, p->perm, p->sbase, p->sfi, h->d_err);
     else
-        k_update<D, 0><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(
-            *c, f ? f->area : nullptr, flux, nullptr, p->csr, p->off, dt, h->d_err);
+        k_update<D><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(
+            *c, f ? f->area : nullptr, flux, p->csr, p->off, dt, h->d_err);
     LAUNCHCHK();
     return 0;
 }
@@ -1138,7 +1220,8 @@ int build_slots(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces*
     int rc = 0;
     long long total = 0;
     do {
-        if (ns > 0) k_plan_slice_width<<<nblocks(ns), PHD_BLOCK, 0, h->stream>>>(*c, p->off, d_w, (int)ns);
+        if (na > 0) k_plan_perm<<<(unsigned)((na + SORT_WINDOW - 1) / SORT_WINDOW), SORT_WINDOW, 0, h->stream>>>(*c, p->off, p->perm);
+        if (ns > 0) k_plan_slice_width<<<nblocks(ns), PHD_BLOCK, 0, h->stream>>>(*c, p->off, p->perm, d_w, (int)ns);
         if (hipPeekAtLastError() != hipSuccess) { (void)hipGetLastError(); rc = PHD_E_HIP; break; }
         if (hipMemcpyAsync(hw, d_w, sizeof(int32_t) * (size_t)ns, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
             hipStreamSynchronize(h->stream) != hipSuccess) { rc = PHD_E_HIP; break; }
@@ -1157,24 +1240,23 @@ int build_slots(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces*
     const size_t s_base = al(sizeof(int32_t) * (size_t)(ns + 1));
     const size_t s_j = al(sizeof(int32_t) * (size_t)total);
     const size_t s_d = al(sizeof(double) * (size_t)total);
-    const size_t s_fs = al(sizeof(int32_t) * 2 * (size_t)nf);
-    const size_t bytes = s_base + s_j + 4 * s_d + NF * s_d + s_fs;
+    const size_t s_g = al(sizeof(double) * (size_t)(NF + (NF & 1)) * (size_t)nf);
+    const size_t bytes = s_base + 2 * s_j + 4 * s_d + s_g;
     if (hipMalloc(&p->slot_base, bytes) != hipSuccess) { free(hw); p->slot_base = nullptr; return PHD_E_HIP; }
     p->bytes += bytes;
     char* b = (char*)p->slot_base;
     p->sbase = (int32_t*)b; b += s_base;
     p->sj = (int32_t*)b; b += s_j;
+    p->sfi = (int32_t*)b; b += s_j;
     for (int q = 0; q < 4; q++) { p->sgeo[q] = (double*)b; b += s_d; }
-    for (int q = 0; q < 5; q++) p->sg[q] = nullptr;
-    for (int q = 0; q < NF; q++) { p->sg[q] = (double*)b; b += s_d; }
-    p->fslot = (int32_t*)b;
+    p->g = (double*)b;
     do {
         if (hipMemcpyAsync(p->sbase, hw, sizeof(int32_t) * (size_t)(ns + 1), hipMemcpyHostToDevice, h->stream) != hipSuccess) { rc = PHD_E_HIP; break; }
         if (total > 0) k_fill_i32<<<nblocks(total), PHD_BLOCK, 0, h->stream>>>(p->sj, total, INVALID);
-        if (nf > 0) k_fill_i32<<<nblocks(2 * nf), PHD_BLOCK, 0, h->stream>>>(p->fslot, 2 * nf, -1);
+        if (total > 0) k_fill_i32<<<nblocks(total), PHD_BLOCK, 0, h->stream>>>(p->sfi, total, INVALID);
         if (na > 0) {
-            if (D == 2) k_plan_slots<2><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->csr, p->off, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->fslot);
-            else k_plan_slots<3><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->csr, p->off, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->fslot);
+            if (D == 2) k_plan_slots<2><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->csr, p->off, p->perm, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->sfi);
+            else k_plan_slots<3><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->csr, p->off, p->perm, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->sfi);
         }
         if (hipPeekAtLastError() != hipSuccess) { (void)hipGetLastError(); rc = PHD_E_HIP; break; }
         if (hipStreamSynchronize(h->stream) != hipSuccess) rc = PHD_E_HIP;
@@ -1294,9 +1376,9 @@ int phd_plan_create(phd_handle* h, const phd_cells* c, const phd_faces* f, phd_p
     const size_t s_csr = al(sizeof(int2) * (size_t)n_entries);
     const size_t s_off = al(sizeof(int32_t) * (size_t)(na + 1));
     const size_t s_xp = al(sizeof(double) * XPW * (size_t)na);
-    const size_t s_g = 0;
+    const size_t s_perm = al(sizeof(int32_t) * (size_t)na);
     const size_t s_bad = al(4 * sizeof(unsigned long long));
-    const size_t total = s_face + s_geo + s_csr + s_off + s_xp + s_g + s_bad;
+    const size_t total = s_face + s_geo + s_csr + s_off + s_xp + s_perm + s_bad;
     phd_plan* p = new phd_plan();
     p->dim = D;
     p->n_cells = (int)na;
@@ -1312,7 +1394,7 @@ int phd_plan_create(phd_handle* h, const phd_cells* c, const phd_faces* f, phd_p
     p->csr = (int2*)b; b += s_csr;
     p->off = (int32_t*)b; b += s_off;
     p->xp = (double*)b; b += s_xp;
-    p->g = nullptr; b += s_g;
+    p->perm = (int32_t*)b; b += s_perm;
     p->d_bad = (unsigned long long*)b;
     p->slot_base = nullptr;
     int rc = 0;
@@ -1327,12 +1409,14 @@ int phd_plan_create(phd_handle* h, const phd_cells* c, const phd_faces* f, phd_p
             if (D == 2) k_plan_cells<2><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, p->cell_geo);
             else k_plan_cells<3><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, p->cell_geo);
             k_plan_csr<<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->csr, p->d_bad);
+            k_plan_runs<<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, p->csr, p->off, p->d_bad + 1);
         }
         if (hipPeekAtLastError() != hipSuccess) { (void)hipGetLastError(); rc = PHD_E_HIP; break; }
-        unsigned long long bad = NOERR;
-        if (hipMemcpyAsync(&bad, p->d_bad, sizeof(bad), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        unsigned long long bad[2] = {NOERR, NOERR};
+        if (hipMemcpyAsync(bad, p->d_bad, sizeof(bad), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
             hipStreamSynchronize(h->stream) != hipSuccess) { rc = PHD_E_HIP; break; }
-        p->bad_cell = bad == NOERR ? -1 : (long long)bad;
+        p->bad_cell = bad[0] == NOERR ? -1 : (long long)bad[0];
+        p->owner_runs = bad[1] == NOERR;
         rc = build_slots(h, p, c, f);
     } while (0);
     if (rc) {
