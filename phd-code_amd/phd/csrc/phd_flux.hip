@@ -74,6 +74,7 @@ struct phd_plan {
     int32_t* sbase;              // n_slices + 1
     int32_t* sj;                 // other cell | incoming bit (INVALID for padding)
     int32_t* sfi;                // face id | incoming bit (INVALID for padding)
+    int2* cinfo;                 // per position: incoming count, first owned face
     double* sgeo[4];             // face centroid x, y, z, area per slot
     double* g;                   // per-face update summand rows (dt*area)*flux, F x GW
     int owner_runs;              // every real cell's owned faces are one id run that
@@ -409,7 +410,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_plan_slots(phd_cells c, phd_faces
                                                           const int32_t* perm,
                                                           const int32_t* sbase, int32_t* sj,
                                                           double* sx, double* sy, double* sz,
-                                                          double* sa, int32_t* sfi)
+                                                          double* sa, int32_t* sfi, int2* cinfo)
 {
     long long t = xcd_index(c.n_cells);
     if (t >= c.n_cells) return;
@@ -417,9 +418,12 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_plan_slots(phd_cells c, phd_faces
     if (c.tag[i] != 0) return;
     const int beg = off[i], end = off[i + 1];
     const long long base = sbase[t / SLICE] + (t % SLICE);
+    int n_in = 0, own0 = -1;
     for (int e = beg; e < end; e++) {
         const int2 en = csr[e];
         const long long slot = base + (long long)(e - beg) * SLICE;
+        if (en.y & INCOMING) n_in++;
+        else if (own0 < 0) own0 = en.x;
         sj[slot] = en.y;
         const int fid = en.x;
         sx[slot] = f.com[0][fid];
@@ -428,6 +432,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_plan_slots(phd_cells c, phd_faces
         sa[slot] = f.area[fid];
         sfi[slot] = (en.y & 0x7fffffff) == INVALID ? INVALID : (fid | (en.y & INCOMING));
     }
+    cinfo[t] = make_int2(n_in, own0);
 }
 
 // ---------------------------------------------------------------------------
@@ -968,9 +973,10 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_update_sell(phd_cells c,
                                                            const int32_t* __restrict__ perm,
                                                            const int32_t* __restrict__ sbase,
                                                            const int32_t* __restrict__ sfi,
+                                                           const int2* __restrict__ cinfo,
                                                            unsigned long long* err)
 {
-    constexpr int NF = D + 2;
+    constexpr int NF = D + 2, GW = summand_width<D>(), B = 4;
     const long long t = xcd_index(c.n_cells);
     if (t >= c.n_cells) return;
     const int i = perm[t];
@@ -981,21 +987,45 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_update_sell(phd_cells c,
     for (int q = 0; q < NF; q++) u[q] = c.cons[q][i];
     const int deg = end - beg;
     const long long base = sbase[t / SLICE] + (t % SLICE);
-    for (int e = 0; e < deg; e++) {
-        const int en = __builtin_nontemporal_load(sfi + base + (long long)e * SLICE);
-        if (en == INVALID) continue;
-        const int fid = en & 0x7fffffff;
-        double s[summand_width<D>()];
-        if (owner_runs && !(en & INCOMING)) {
-            // the rest of the list: owned faces fid, fid+1, ..
-            for (int k = 0; k < deg - e; k++) {
-                load_summand<D>(g, fid + k, s);
-                if (apply_summand<D>(u, s, false, fid + k, err)) break;
-            }
-            break;
+    if (owner_runs) {
+        // incoming rows (gathered) then the owned run, B rows in flight at a
+        // time; applied one by one in list order
+        const int2 ci = cinfo[t];
+        const int n_in = ci.x, n_own = deg - ci.x;
+        bool stop = false;
+        for (int e0 = 0; e0 < n_in && !stop; e0 += B) {
+            int fb[B];
+            double sb[B][GW];
+#pragma unroll
+            for (int b = 0; b < B; b++)
+                fb[b] = e0 + b < n_in
+                            ? (__builtin_nontemporal_load(sfi + base + (long long)(e0 + b) * SLICE) & 0x7fffffff)
+                            : -1;
+#pragma unroll
+            for (int b = 0; b < B; b++)
+                if (fb[b] >= 0) load_summand<D>(g, fb[b], sb[b]);
+#pragma unroll
+            for (int b = 0; b < B; b++)
+                if (!stop && fb[b] >= 0) stop = apply_summand<D>(u, sb[b], true, fb[b], err);
         }
-        load_summand<D>(g, fid, s);
-        if (apply_summand<D>(u, s, en & INCOMING, fid, err)) break;
+        for (int k0 = 0; k0 < n_own && !stop; k0 += B) {
+            double sb[B][GW];
+#pragma unroll
+            for (int b = 0; b < B; b++)
+                if (k0 + b < n_own) load_summand<D>(g, ci.y + k0 + b, sb[b]);
+#pragma unroll
+            for (int b = 0; b < B; b++)
+                if (!stop && k0 + b < n_own) stop = apply_summand<D>(u, sb[b], false, ci.y + k0 + b, err);
+        }
+    } else {
+        for (int e = 0; e < deg; e++) {
+            const int en = __builtin_nontemporal_load(sfi + base + (long long)e * SLICE);
+            if (en == INVALID) continue;
+            const int fid = en & 0x7fffffff;
+            double s[GW];
+            load_summand<D>(g, fid, s);
+            if (apply_summand<D>(u, s, en & INCOMING, fid, err)) break;
+        }
     }
 #pragma unroll
     for (int q = 0; q < NF; q++) c.cons[q][i] = u[q];
@@ -1198,7 +1228,7 @@ int launch_update(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_face
     if (c->n_cells == 0) return 0;
     if (SRC == 1)
         k_update_sell<D><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(
-            *c, p->g, p->owner_runs, p->csr, p->off, p->perm, p->sbase, p->sfi, h->d_err);
+            *c, p->g, p->owner_runs, p->csr, p->off, p->perm, p->sbase, p->sfi, p->cinfo, h->d_err);
     else
         k_update<D><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(
             *c, f ? f->area : nullptr, flux, p->csr, p->off, dt, h->d_err);
@@ -1241,7 +1271,8 @@ int build_slots(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces*
     const size_t s_j = al(sizeof(int32_t) * (size_t)total);
     const size_t s_d = al(sizeof(double) * (size_t)total);
     const size_t s_g = al(sizeof(double) * (size_t)(NF + (NF & 1)) * (size_t)nf);
-    const size_t bytes = s_base + 2 * s_j + 4 * s_d + s_g;
+    const size_t s_ci = al(sizeof(int2) * (size_t)na);
+    const size_t bytes = s_base + 2 * s_j + 4 * s_d + s_g + s_ci;
     if (hipMalloc(&p->slot_base, bytes) != hipSuccess) { free(hw); p->slot_base = nullptr; return PHD_E_HIP; }
     p->bytes += bytes;
     char* b = (char*)p->slot_base;
@@ -1249,14 +1280,15 @@ int build_slots(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces*
     p->sj = (int32_t*)b; b += s_j;
     p->sfi = (int32_t*)b; b += s_j;
     for (int q = 0; q < 4; q++) { p->sgeo[q] = (double*)b; b += s_d; }
-    p->g = (double*)b;
+    p->g = (double*)b; b += s_g;
+    p->cinfo = (int2*)b;
     do {
         if (hipMemcpyAsync(p->sbase, hw, sizeof(int32_t) * (size_t)(ns + 1), hipMemcpyHostToDevice, h->stream) != hipSuccess) { rc = PHD_E_HIP; break; }
         if (total > 0) k_fill_i32<<<nblocks(total), PHD_BLOCK, 0, h->stream>>>(p->sj, total, INVALID);
         if (total > 0) k_fill_i32<<<nblocks(total), PHD_BLOCK, 0, h->stream>>>(p->sfi, total, INVALID);
         if (na > 0) {
-            if (D == 2) k_plan_slots<2><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->csr, p->off, p->perm, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->sfi);
-            else k_plan_slots<3><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->csr, p->off, p->perm, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->sfi);
+            if (D == 2) k_plan_slots<2><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->csr, p->off, p->perm, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->sfi, p->cinfo);
+            else k_plan_slots<3><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->csr, p->off, p->perm, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->sfi, p->cinfo);
         }
         if (hipPeekAtLastError() != hipSuccess) { (void)hipGetLastError(); rc = PHD_E_HIP; break; }
         if (hipStreamSynchronize(h->stream) != hipSuccess) rc = PHD_E_HIP;
