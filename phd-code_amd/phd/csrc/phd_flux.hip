@@ -33,78 +33,12 @@
 // expression follows the reference's order, so results are bit-identical to
 // its x86-64 build except where libm pow() and the GPU's pow() round
 // differently.
-#include <hip/hip_runtime.h>
-#include <stdint.h>
 #include <string.h>
-#include "phd_flux.h"
-#include "riemann_dev.h"
-#include "states_dev.h"
-
-#define PHD_BLOCK 256
-
-struct phd_handle {
-    int device;
-    hipStream_t stream;
-    int mode;
-    unsigned long long* d_err;   // [0] update key, [1] neighbor cell, [2] exact face, [3] dt bits
-    unsigned long long* h_err;   // pinned host mirror
-};
-
-struct phd_plan {
-    int dim, n_cells, n_faces;
-    long long n_entries;
-    int device;
-    void* base;                  // single allocation holding every buffer below
-    size_t bytes;
-    double* face_rec;
-    double* cell_geo;
-    int2* csr;
-    int32_t* off;
-    double* xp;
-    unsigned long long* d_bad;
-    long long bad_cell;          // first real cell with an inconsistent face list, or -1
-    // sliced lists: cells are visited in the order perm (within each window
-    // of 256 consecutive cells, by descending face count; reals before
-    // ghosts), and the cell at position t keeps entry r of its face list in
-    // slot sbase[t/16] + 16 r + t%16.  A wavefront reads 4 x 16 consecutive
-    // slots per entry, and its 64 lanes have nearly equal list lengths.
-    int32_t* perm;               // position -> cell, n_cells
-    long long n_slots;
-    void* slot_base;             // second allocation holding the slot arrays
-    int32_t* sbase;              // n_slices + 1
-    int32_t* sj;                 // other cell | incoming bit (INVALID for padding)
-    int32_t* sfi;                // face id | incoming bit (INVALID for padding)
-    int2* cinfo;                 // per position: incoming count, first owned face
-    double* sgeo[4];             // face centroid x, y, z, area per slot
-    double* g;                   // per-face update summand rows (dt*area)*flux, F x GW
-    int owner_runs;              // every real cell's owned faces are one id run that
-                                 // follows all its incoming faces (faces in owner order)
-};
+#include "flux_common.h"
 
 using namespace phd;
 
 namespace {
-
-constexpr unsigned long long NOERR = ~0ull;
-constexpr unsigned long long INF_BITS = 0x7ff0000000000000ull;
-constexpr int INCOMING = (int)0x80000000u;
-constexpr int INVALID = 0x7fffffff;
-constexpr int FREC = 8;      // doubles per face record
-constexpr int GEO = 4;       // doubles per cell geometry row
-constexpr int XPW = 8;       // doubles per packed cell row
-
-struct FaceArrays { double* q[5]; };
-
-// doubles per summand row: D+2 padded to even (16-byte aligned rows)
-template <int D> constexpr int summand_width() { return (D + 3) & ~1; }
-
-template <int D> struct Lay {
-    static constexpr int NF = D + 2;                 // primitive fields
-    static constexpr int G = D * (D + 2);            // gradient components
-    static constexpr int REC = NF + G + D;           // used doubles per record row
-    static constexpr int LD = (REC + 1) & ~1;        // 24 (3D) / 14 (2D)
-    static constexpr int P0 = 0, G0 = NF, C0 = NF + G;
-};
 
 __global__ void k_reset_err(unsigned long long* e)
 {
@@ -117,26 +51,6 @@ __global__ void k_set_err(unsigned long long* e, int slot, unsigned long long v)
     if (threadIdx.x == 0) atomicMin(&e[slot], v);
 }
 
-inline unsigned nblocks(long long n) { return (unsigned)((n + PHD_BLOCK - 1) / PHD_BLOCK); }
-// grid for an XCD-remapped launch over n items: a multiple of 8 blocks
-inline unsigned xcd_grid(long long n) { unsigned nb = nblocks(n); return 8u * ((nb + 7u) / 8u); }
-
-// logical item index of this thread under the XCD remap; n items in total
-__device__ __forceinline__ long long xcd_index(long long n)
-{
-    const long long nb = (n + PHD_BLOCK - 1) / PHD_BLOCK;
-    const long long per = (nb + 7) / 8;
-    const long long b = blockIdx.x;
-    const long long L = (b & 7) * per + (b >> 3);
-    return L * PHD_BLOCK + threadIdx.x;
-}
-
-__device__ __forceinline__ void ld2(const double* p, double& a, double& b)
-{
-    double2 v = *reinterpret_cast<const double2*>(p);
-    a = v.x;
-    b = v.y;
-}
 
 // ---------------------------------------------------------------------------
 // equation of state (equation_state.pyx:43-117), all cells
@@ -726,59 +640,6 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_ghost_rec(phd_cells c, const doub
 
 // ---------------------------------------------------------------------------
 // face kernels
-template <int D>
-__device__ __forceinline__ void load_face_rec(const double* frec, long long m, double* com,
-                                              double& area, double* n, double& diff)
-{
-    double r[FREC];
-#pragma unroll
-    for (int q = 0; q < FREC / 2; q++) ld2(frec + (size_t)m * FREC + 2 * q, r[2 * q], r[2 * q + 1]);
-#pragma unroll
-    for (int k = 0; k < D; k++) { com[k] = r[k]; n[k] = r[4 + k]; }
-    area = r[3];
-    diff = r[7];
-}
-
-template <int D>
-__device__ __forceinline__ void load_rec(const double* rec, int ld, int i, double* row)
-{
-    using Y = Lay<D>;
-    const double* p = rec + (size_t)i * ld;
-#pragma unroll
-    for (int q = 0; q < Y::LD / 2; q++) ld2(p + 2 * q, row[2 * q], row[2 * q + 1]);
-}
-
-// left/right states of face m into L, R (primitive group, face frame)
-template <int D, int ORDER>
-__device__ __forceinline__ void face_states(const phd_cells& c, const double* rec, int ld, int i,
-                                            int j, const double* com, double diff,
-                                            const double* fv, double gamma, double dt, int boost,
-                                            int gizmo, int stages, double* L, double* R)
-{
-    using Y = Lay<D>;
-    if (ORDER == PHD_ORDER_CONSTANT) {
-        double pi[Y::NF], pj[Y::NF];
-#pragma unroll
-        for (int n = 0; n < Y::NF; n++) { pi[n] = c.prim[n][i]; pj[n] = c.prim[n][j]; }
-        boost_copy<D>(pi, pj, fv, boost, L, R);
-        return;
-    }
-    double ri[Y::LD], rj[Y::LD];
-    load_rec<D>(rec, ld, i, ri);
-    load_rec<D>(rec, ld, j, rj);
-    const double* pi = ri + Y::P0;
-    const double* pj = rj + Y::P0;
-    if (stages & PHD_STAGE_SPATIAL) {
-        boost_copy<D>(pi, pj, fv, boost, L, R);
-        spatial_states<D>(com, diff, ri + Y::C0, rj + Y::C0, pi, pj, ri + Y::G0, rj + Y::G0,
-                          gizmo, L, R);
-    }
-    if (stages & PHD_STAGE_TEMPORAL) {
-        temporal_side<D>(pi, ri + Y::G0, fv, boost, gamma, dt, L);
-        temporal_side<D>(pj, rj + Y::G0, fv, boost, gamma, dt, R);
-    }
-}
-
 // Materialised states: PieceWiseConstant.compute_states or
 // PieceWiseLinear.add_spatial / add_temporal.
 template <int D, int ORDER>

@@ -6,6 +6,13 @@
 // reference's x86-64 build.
 #pragma once
 #include <hip/hip_runtime.h>
+#include "phd_flux.h"
+
+// callable from kernels and from host code (the host build runs the same
+// functions for the CPU checks of the adjoint)
+#ifndef PHD_HD
+#define PHD_HD __host__ __device__
+#endif
 
 namespace phd {
 
@@ -36,7 +43,7 @@ inline Gas make_gas(double gamma)
 // a base of exactly one changes no bit.  In a uniform region every TRRS base
 // is exactly one (equal pressures, equal velocities), and a wave whose lanes
 // all skip never executes the pow sequence.
-__device__ __forceinline__ double pow_b1(double x, double y)
+PHD_HD __forceinline__ double pow_b1(double x, double y)
 {
     double r = 1.0;
     if (__builtin_expect(x != 1.0, 0)) r = pow(x, y);
@@ -45,7 +52,7 @@ __device__ __forceinline__ double pow_b1(double x, double y)
 
 // HLL.get_waves (riemann.pyx:375-504): Toro PVRS / TRRS / TSRS p* estimate
 // and Toro eq. 10.68-10.70 wave speeds.
-__device__ __forceinline__ void get_waves(double dl, double ul, double pl, double dr, double ur,
+PHD_HD __forceinline__ void get_waves(double dl, double ul, double pl, double dr, double ur,
                                           double pr, const Gas& g, double& sl, double& sc,
                                           double& sr)
 {
@@ -84,7 +91,7 @@ __device__ __forceinline__ void get_waves(double dl, double ul, double pl, doubl
 
 // pure upwind state flux in the moving frame (riemann.pyx:624-631, 672-679)
 template <int D>
-__device__ __forceinline__ void side_flux(const double* S, const double* n, double Vn, double vsq,
+PHD_HD __forceinline__ void side_flux(const double* S, const double* n, double Vn, double vsq,
                                           double wn, double gamma, double* F)
 {
     double d = S[0], p = S[D + 1];
@@ -96,7 +103,7 @@ __device__ __forceinline__ void side_flux(const double* S, const double* n, doub
 
 // HLL (riemann.pyx:243-373) and HLLC (riemann.pyx:521-682) interface flux
 template <int D, int SOLVER>
-__device__ __forceinline__ void riemann_hll(const double* L, const double* R, const double* n,
+PHD_HD __forceinline__ void riemann_hll(const double* L, const double* R, const double* n,
                                             const double* fv, const Gas& g, int boost, double* F)
 {
     const double gamma = g.gamma;
@@ -159,7 +166,7 @@ __device__ __forceinline__ void riemann_hll(const double* L, const double* R, co
 
 // RiemannBase.deboost (riemann.pyx:190-222), Pakmor & Springel (2011) eq. 17
 template <int D>
-__device__ __forceinline__ void deboost(const double* fv, double* F)
+PHD_HD __forceinline__ void deboost(const double* fv, double* F)
 {
 #pragma unroll
     for (int k = 0; k < D; k++) {
@@ -169,7 +176,7 @@ __device__ __forceinline__ void deboost(const double* fv, double* F)
 }
 
 // ---- Exact solver (riemann.pyx:684-1090), Toro chapter 4 -------------------
-__device__ inline double ex_p_func(double d, double p, double c, double gamma, double p_old)
+PHD_HD inline double ex_p_func(double d, double p, double c, double gamma, double p_old)
 {
     if (p_old <= p) return 2. * c / (gamma - 1.) * (pow_b1(p_old / p, (gamma - 1.) / (2. * gamma)) - 1.);
     double Ak = 2. / (d * (gamma + 1.));
@@ -177,7 +184,7 @@ __device__ inline double ex_p_func(double d, double p, double c, double gamma, d
     return (p_old - p) * sqrt(Ak / (p_old + Bk));
 }
 
-__device__ inline double ex_p_deriv(double d, double p, double c, double gamma, double p_old)
+PHD_HD inline double ex_p_deriv(double d, double p, double c, double gamma, double p_old)
 {
     if (p_old <= p) return pow_b1(p_old / p, -(gamma + 1.) / (2. * gamma)) / (c * d);
     double Ak = 2. / (d * (gamma + 1.));
@@ -186,7 +193,7 @@ __device__ inline double ex_p_deriv(double d, double p, double c, double gamma, 
 }
 
 // p_guess + get_pstar (riemann.pyx:975-1090): Newton, TOL 1e-6, 1000 steps
-__device__ inline bool ex_pstar(double dl, double ul, double pl, double cl, double dr, double ur,
+PHD_HD inline bool ex_pstar(double dl, double ul, double pl, double cl, double dr, double ur,
                                 double pr, double cr, double gamma, double& out)
 {
     double ppv = .5 * (pl + pr) - .125 * (ur - ul) * (dl + dr) * (cl + cr);
@@ -222,7 +229,7 @@ __device__ inline bool ex_pstar(double dl, double ul, double pl, double cl, doub
 }
 
 template <int D>
-__device__ __forceinline__ void ex_state_flux(double d, double p, const double* v, const double* n,
+PHD_HD __forceinline__ void ex_state_flux(double d, double p, const double* v, const double* n,
                                               double gamma, double* F)
 {
     double v_sq = 0., vn = 0.;
@@ -237,7 +244,7 @@ __device__ __forceinline__ void ex_state_flux(double d, double p, const double* 
 // Exact.riemann_solver sampling at x/t = 0 (riemann.pyx:763-970).  Returns
 // false on Newton non-convergence.
 template <int D>
-__device__ bool riemann_exact(const double* L, const double* R, const double* n, double gamma,
+PHD_HD bool riemann_exact(const double* L, const double* R, const double* n, double gamma,
                               double* F)
 {
     double _dl = L[0], _pl = L[D + 1], _dr = R[0], _pr = R[D + 1];
@@ -324,7 +331,7 @@ __device__ bool riemann_exact(const double* L, const double* R, const double* n,
 // Riemann solve + deboost as RiemannBase.compute_fluxes does (riemann.pyx:74-98);
 // Exact always deboosts (972).
 template <int D, int SOLVER>
-__device__ __forceinline__ bool solve_face(const double* L, const double* R, const double* n,
+PHD_HD __forceinline__ bool solve_face(const double* L, const double* R, const double* n,
                                            const double* fv, const Gas& g, int boost, double* F)
 {
     if (SOLVER == PHD_SOLVER_EXACT) {

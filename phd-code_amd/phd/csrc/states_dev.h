@@ -2,12 +2,18 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// callable from kernels and from host code (the host build runs the same
+// functions for the CPU checks of the adjoint)
+#ifndef PHD_HD
+#define PHD_HD __host__ __device__
+#endif
+
 namespace phd {
 
 // face-frame copies of the cell states (reconstruction.pyx:838-852; the
 // constant reconstruction 330-352 is the same copy)
 template <int D>
-__device__ __forceinline__ void boost_copy(const double* pi, const double* pj, const double* fv,
+PHD_HD __forceinline__ void boost_copy(const double* pi, const double* pj, const double* fv,
                                            int boost, double* L, double* R)
 {
 #pragma unroll
@@ -24,7 +30,7 @@ __device__ __forceinline__ void boost_copy(const double* pi, const double* pj, c
 //                        (the square roots are taken here, as in the reference)
 //   sl, sr             : the face-frame cell states before extrapolation
 template <int D>
-__device__ __forceinline__ void gizmo_limit(double sepi_mag, double sepj_mag, double diff,
+PHD_HD __forceinline__ void gizmo_limit(double sepi_mag, double sepj_mag, double diff,
                                             const double* pi, const double* pj, const double* sl,
                                             const double* sr, double* L, double* R)
 {
@@ -72,7 +78,7 @@ __device__ __forceinline__ void gizmo_limit(double sepi_mag, double sepj_mag, do
 //   diff  : |x_j - x_i| (866, 875)
 //   pi,pj : lab-frame primitives (the GIZMO branch test at 902/906)
 template <int D>
-__device__ __forceinline__ void spatial_states(const double* com, double diff, const double* ci,
+PHD_HD __forceinline__ void spatial_states(const double* com, double diff, const double* ci,
                                                const double* cj, const double* pi,
                                                const double* pj, const double* gi,
                                                const double* gj, int gizmo, double* L, double* R)
@@ -101,7 +107,7 @@ __device__ __forceinline__ void spatial_states(const double* com, double diff, c
 // half-step primitive Euler predictor in the face frame.  Gradient groups:
 // d rho/dx_k = g[k], d v_q/dx_k = g[(1+q)D+k], d p/dx_k = g[(D+1)D+k].
 template <int D>
-__device__ __forceinline__ void temporal_side(const double* p, const double* g, const double* fv,
+PHD_HD __forceinline__ void temporal_side(const double* p, const double* g, const double* fv,
                                               int boost, double gamma, double dt, double* S)
 {
     constexpr int NF = D + 2;

@@ -1,0 +1,294 @@
+"""CPU checks of the hand-written reverse mode (csrc/adjoint_dev.h).
+
+The device functions are compiled for the host (tests/adjoint/adjoint_host.cpp)
+together with the forward functions they differentiate.  For each piece,
+seeded random inputs, a random output weight w and a random direction v give
+the dot-product test
+
+    w . (f(x + e v) - f(x - e v)) / (2 e)  ==  (J^T w) . v
+
+to 1e-6 relative.  The pieces are piecewise smooth (limiters, wave-speed
+branches): a sample whose two one-sided slopes disagree straddles a branch
+switch and is skipped; at most a few percent may be.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "adjoint", "adjoint_host.cpp")
+LIB = os.path.join(HERE, "adjoint", "libadjoint_host.so")
+CSRC = os.path.join(ROOT, "phd-code_amd", "phd", "csrc")
+DEPS = [SRC] + [os.path.join(CSRC, f) for f in ("adjoint_dev.h", "riemann_dev.h", "states_dev.h")]
+
+P = ctypes.POINTER(ctypes.c_double)
+
+
+def _lib():
+    if not os.path.exists(LIB) or os.path.getmtime(LIB) < max(os.path.getmtime(d) for d in DEPS):
+        hipcc = "/opt/rocm/bin/hipcc"
+        cmd = [hipcc, "-O2", "-std=c++17", "-ffp-contract=off", "--offload-arch=gfx950", "-fPIC",
+               "-shared", "-I" + CSRC, "-I" + os.path.join(ROOT, "include"), "-x", "hip", SRC,
+               "-o", LIB]
+        subprocess.run(cmd, check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    return ctypes.CDLL(LIB)
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return _lib()
+
+
+def _p(a):
+    return a.ctypes.data_as(P) if a is not None else None
+
+
+def _dot_test(f, vjp, x, rng, n_out, eps_rel=1e-6, tol=1e-6):
+    """Returns (ok, skipped) for one sample."""
+    w = rng.normal(size=n_out)
+    v = rng.normal(size=x.size)
+    e = eps_rel * max(1.0, np.abs(x).max())
+    y0, yp, ym = f(x), f(x + e * v), f(x - e * v)
+    sp, sm = w @ (yp - y0) / e, w @ (y0 - ym) / e
+    fd = w @ (yp - ym) / (2 * e)
+    scale = max(abs(sp), abs(sm), 1e-30)
+    if abs(sp - sm) > 1e-2 * scale + 1e-12:
+        return True, True                      # a branch switch between the points
+    adj = vjp(x, w) @ v
+    return abs(fd - adj) <= tol * max(abs(fd), abs(adj), 1e-8 * np.abs(w).sum()), False
+
+
+def _run(samples, f, vjp, n_out, gen, seed):
+    rng = np.random.default_rng(seed)
+    bad, skip = [], 0
+    for s in range(samples):
+        x = gen(rng)
+        ok, skipped = _dot_test(f, vjp, x, rng, n_out)
+        skip += skipped
+        if not ok:
+            bad.append(s)
+    assert not bad, "VJP mismatch on samples %s" % bad[:10]
+    assert skip <= 0.05 * samples, "too many kinked samples: %d" % skip
+
+
+def _state(rng, dim):
+    return np.concatenate([[10 ** rng.uniform(-1, 1)], rng.uniform(-1, 1, dim),
+                           [10 ** rng.uniform(-1, 1)]])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("moving", [False, True])
+def test_hllc_vjp(lib, dim, moving):
+    nf = dim + 2
+    rng0 = np.random.default_rng(1)
+    n = rng0.normal(size=dim)
+    n /= np.linalg.norm(n)
+    fv = rng0.uniform(-0.3, 0.3, dim) if moving else np.zeros(dim)
+    gamma = 1.4
+
+    def f(x):
+        F = np.zeros(nf)
+        lib.ah_hllc(dim, _p(x[:nf].copy()), _p(x[nf:].copy()), _p(n), _p(fv),
+                    ctypes.c_double(gamma), None, _p(F), None, None)
+        return F
+
+    def vjp(x, w):
+        lL, lR = np.zeros(nf), np.zeros(nf)
+        lib.ah_hllc(dim, _p(x[:nf].copy()), _p(x[nf:].copy()), _p(n), _p(fv),
+                    ctypes.c_double(gamma), _p(w), None, _p(lL), _p(lR))
+        return np.concatenate([lL, lR])
+
+    _run(400, f, vjp, nf, lambda r: np.concatenate([_state(r, dim), _state(r, dim)]), 3)
+
+
+def test_hllc_vjp_wave_branches(lib):
+    """Inputs chosen to reach each p* estimate (PVRS, TRRS, TSRS) and each
+    flux branch (left/right supersonic, left/right star)."""
+    dim, nf, gamma = 2, 4, 1.4
+    n, fv = np.array([1.0, 0.0]), np.zeros(2)
+    cases = [
+        ([1.0, 0.1, 0.0, 1.0], [0.9, 0.05, 0.0, 0.95]),     # PVRS, weak
+        ([1.0, -1.0, 0.0, 1.0], [0.8, 0.9, 0.0, 0.7]),      # TRRS (rarefactions)
+        ([1.0, 2.0, 0.0, 1.0], [0.8, -1.5, 0.0, 1.2]),      # TSRS (colliding)
+        ([1.0, 4.0, 0.2, 1.0], [0.5, 3.0, 0.1, 0.5]),       # supersonic right-moving
+        ([1.0, -4.0, 0.2, 1.0], [0.5, -3.0, 0.1, 0.5]),     # supersonic left-moving
+        ([1.0, 0.3, 0.1, 2.0], [0.2, 0.1, 0.0, 0.3]),       # left star
+        ([0.2, -0.3, 0.1, 0.3], [1.0, -0.1, 0.0, 2.0]),     # right star
+    ]
+
+    def f(x):
+        F = np.zeros(nf)
+        lib.ah_hllc(dim, _p(x[:nf].copy()), _p(x[nf:].copy()), _p(n), _p(fv),
+                    ctypes.c_double(gamma), None, _p(F), None, None)
+        return F
+
+    def vjp(x, w):
+        lL, lR = np.zeros(nf), np.zeros(nf)
+        lib.ah_hllc(dim, _p(x[:nf].copy()), _p(x[nf:].copy()), _p(n), _p(fv),
+                    ctypes.c_double(gamma), _p(w), None, _p(lL), _p(lR))
+        return np.concatenate([lL, lR])
+
+    for L, R in cases:
+        # full Jacobian by central differences against the reverse mode
+        x = np.array(L + R, dtype=np.float64)
+        J = np.zeros((nf, 2 * nf))
+        for c in range(2 * nf):
+            d = np.zeros(2 * nf)
+            d[c] = 1e-6
+            J[:, c] = (f(x + d) - f(x - d)) / 2e-6
+        JA = np.array([vjp(x, np.eye(nf)[q]) for q in range(nf)])
+        np.testing.assert_allclose(JA, J, rtol=0, atol=1e-7 * np.abs(J).max(), err_msg=str((L, R)))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_temporal_vjp(lib, dim):
+    nf, G = dim + 2, dim * (dim + 2)
+    rng0 = np.random.default_rng(4)
+    fv = rng0.uniform(-0.2, 0.2, dim)
+
+    def split(x):
+        return x[:nf].copy(), x[nf:nf + G].copy(), x[nf + G:].copy()
+
+    def f(x):
+        p, g, S = split(x)
+        out = np.zeros(nf)
+        lib.ah_temporal(dim, _p(p), _p(g), _p(fv), 1, ctypes.c_double(1.4), ctypes.c_double(0.01),
+                        _p(S), None, _p(out), None, None, None)
+        return out
+
+    def vjp(x, w):
+        p, g, S = split(x)
+        lS, lp, lg = np.zeros(nf), np.zeros(nf), np.zeros(G)
+        lib.ah_temporal(dim, _p(p), _p(g), _p(fv), 1, ctypes.c_double(1.4), ctypes.c_double(0.01),
+                        _p(S), _p(w), None, _p(lS), _p(lp), _p(lg))
+        return np.concatenate([lp, lg, lS])
+
+    def gen(r):
+        p = _state(r, dim)
+        g = r.normal(size=G)
+        S = p + 0.05 * r.normal(size=nf)
+        S[0], S[-1] = abs(S[0]), abs(S[-1])
+        return np.concatenate([p, g, S])
+
+    _run(300, f, vjp, nf, gen, 5)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("gizmo", [True, False])
+def test_spatial_vjp(lib, dim, gizmo):
+    nf, G = dim + 2, dim * (dim + 2)
+    rng0 = np.random.default_rng(6)
+    ci = rng0.uniform(0, 1, dim)
+    cj = ci + rng0.uniform(0.05, 0.1, dim)
+    com = 0.5 * (ci + cj) + rng0.uniform(-0.01, 0.01, dim)
+    diff = np.linalg.norm(cj - ci) * 1.02
+    fv = rng0.uniform(-0.2, 0.2, dim)
+
+    def split(x):
+        return (x[:nf].copy(), x[nf:2 * nf].copy(), x[2 * nf:2 * nf + G].copy(),
+                x[2 * nf + G:].copy())
+
+    def f(x):
+        pi, pj, gi, gj = split(x)
+        L, R = np.zeros(nf), np.zeros(nf)
+        lib.ah_spatial(dim, _p(com), ctypes.c_double(diff), _p(ci), _p(cj), _p(pi), _p(pj),
+                       _p(gi), _p(gj), _p(fv), 1, int(gizmo), None, None, _p(L), _p(R),
+                       None, None, None, None)
+        return np.concatenate([L, R])
+
+    def vjp(x, w):
+        pi, pj, gi, gj = split(x)
+        lpi, lpj, lgi, lgj = np.zeros(nf), np.zeros(nf), np.zeros(G), np.zeros(G)
+        lib.ah_spatial(dim, _p(com), ctypes.c_double(diff), _p(ci), _p(cj), _p(pi), _p(pj),
+                       _p(gi), _p(gj), _p(fv), 1, int(gizmo), _p(w[:nf].copy()),
+                       _p(w[nf:].copy()), None, None, _p(lpi), _p(lpj), _p(lgi), _p(lgj))
+        return np.concatenate([lpi, lpj, lgi, lgj])
+
+    def gen(r):
+        return np.concatenate([_state(r, dim), _state(r, dim), 3.0 * r.normal(size=2 * G)])
+
+    _run(400, f, vjp, 2 * nf, gen, 7)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("to_prim", [0, 1])
+def test_eos_vjp(lib, dim, to_prim):
+    nf = dim + 2
+    vol = 0.37
+
+    def f(x):
+        y = np.zeros(nf)
+        lib.ah_eos(dim, to_prim, _p(x.copy()), ctypes.c_double(vol), ctypes.c_double(1.4), None,
+                   _p(y), None)
+        return y
+
+    def vjp(x, w):
+        lx = np.zeros(nf)
+        lib.ah_eos(dim, to_prim, _p(x.copy()), ctypes.c_double(vol), ctypes.c_double(1.4),
+                   _p(w), None, _p(lx))
+        return lx
+
+    def gen(r):
+        s = _state(r, dim)
+        if to_prim:   # conserved state of that primitive state
+            return f_c(s)
+        return s
+
+    def f_c(s):
+        y = np.zeros(nf)
+        lib.ah_eos(dim, 0, _p(s.copy()), ctypes.c_double(vol), ctypes.c_double(1.4), None,
+                   _p(y), None)
+        return y
+
+    _run(200, f, vjp, nf, gen, 8)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_grad_cell_vjp(lib, dim):
+    """eq. 21 gradient + AREPO limiter of one cell w.r.t. its own and its
+    neighbours' primitives (geometry fixed)."""
+    nf, G = dim + 2, dim * (dim + 2)
+    rng0 = np.random.default_rng(12)
+    deg = 7 if dim == 2 else 14
+    xi = rng0.uniform(0.4, 0.6, dim)
+    dirs = rng0.normal(size=(deg, dim))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    xj = xi + 0.1 * dirs * rng0.uniform(0.8, 1.2, (deg, 1))
+    com = np.zeros((deg, 3))
+    com[:, :dim] = 0.5 * (xi + xj) + 0.01 * rng0.normal(size=(deg, dim))
+    area = rng0.uniform(0.02, 0.05, deg)
+    cx = xi + 0.005 * rng0.normal(size=dim)
+    vol = 0.01
+
+    def unpack(x):
+        pv = x[:nf].copy()
+        rows = np.zeros((deg, 8))
+        rows[:, :dim] = xj
+        rows[:, dim:dim + nf] = x[nf:].reshape(deg, nf)
+        return pv, rows
+
+    def f(x):
+        pv, rows = unpack(x)
+        grad = np.zeros(G)
+        lib.ah_grad_cell(dim, deg, _p(rows.ravel().copy()), _p(com.ravel().copy()), _p(area),
+                         _p(pv), _p(xi), _p(cx), ctypes.c_double(vol), None, _p(grad), None, None)
+        return grad
+
+    def vjp(x, w):
+        pv, rows = unpack(x)
+        lself, lnb = np.zeros(nf), np.zeros(deg * nf)
+        lib.ah_grad_cell(dim, deg, _p(rows.ravel().copy()), _p(com.ravel().copy()), _p(area),
+                         _p(pv), _p(xi), _p(cx), ctypes.c_double(vol), _p(w.copy()), None,
+                         _p(lself), _p(lnb))
+        return np.concatenate([lself, lnb])
+
+    def gen(r):
+        pv = _state(r, dim)
+        nb = np.array([pv + 0.3 * r.normal(size=nf) * np.abs(pv) for _ in range(deg)])
+        return np.concatenate([pv, nb.ravel()])
+
+    _run(300, f, vjp, G, gen, 13)
