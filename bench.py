@@ -11,7 +11,9 @@ A step is one GPU hydro step on that mesh, all inputs resident in HBM:
   MovingMeshMUSCLHancock.evolve_timestep, integrate.py:396-408), then the
   ghost conserved refresh + EOS epilogue (integrate.py:371-377) so the state
   stays physical from step to step.
-value = faces / (wall time per step).  Per-kernel HIP-event times give the
+value = faces / (wall time per step).  The adjoint block times the forward
+and reverse of a checkpointed KH 256^2 rollout (50 steps, a checkpoint every
+10) and reports reverse/forward.  Per-kernel HIP-event times give the
 roofline line for the dominant kernel.  The CPU baseline is the oracle
 (serial C restatement of the reference loops) timed on one host core on one
 full core step of the same workload.
@@ -166,6 +168,45 @@ def cpu_baseline(problem, dt):
                       (len(times), a["nf"], ", ".join("%.2f" % x for x in times))}
 
 
+def adjoint_timing(n=256, n_steps=50, every=10, reps=3):
+    """Forward and reverse of a checkpointed KH rollout on a static mesh:
+    n^2 cells, n_steps steps, a checkpoint every `every` steps, gradient of
+    an L2 density misfit with respect to the initial density and v_y.  The
+    forward is the rollout's own (steps + checkpoints, time steps fixed to
+    the CFL steps of a first run); the backward replays each segment from
+    its checkpoint and runs the step VJPs in reverse."""
+    import torch
+    from phd import problems
+    from phd.adjoint import StaticStepper, forward_dts, rollout
+    st = StaticStepper(problems.kh_2d(n=n, seed=0))
+    real = st.real
+    pc = st.p.particles
+    rho0 = pc["density"][real].clone()
+    vy0 = pc["velocity-y"][real].clone()
+    _, dts = forward_dts(st, rho0, vy0, n_steps)
+    target = rho0.clone()
+    fwd, bwd = [], []
+    for _ in range(reps + 1):
+        r0 = rho0.clone().requires_grad_(True)
+        v0 = vy0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rho = rollout(st, r0, v0, n_steps, every=every, dts=dts)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ((rho - target) ** 2).sum().backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fwd.append(t1 - t0)
+        bwd.append(t2 - t1)
+    fwd, bwd = sorted(fwd[1:])[reps // 2], sorted(bwd[1:])[reps // 2]
+    st.close()
+    return {"workload": "KH %d^2 static mesh, %d steps, checkpoint every %d, HLLC + "
+                        "PieceWiseLinear(arepo, gizmo)" % (n, n_steps, every),
+            "forward_ms": round(fwd * 1e3, 3), "backward_ms": round(bwd * 1e3, 3),
+            "ratio": round(bwd / fwd, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +214,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write the traffic record from two rocprofv3 --pmc runs of this "
                          "benchmark (FETCH_SIZE, WRITE_SIZE) and exit; no GPU needed")
@@ -310,7 +352,11 @@ def main():
         "core_model_frac_hbm": round(B["core_model"] / (core_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,
         "cpu_baseline": None,
+        "adjoint": None,
     }
+    if not args.no_adjoint:
+        result["adjoint"] = adjoint_timing()
+        result["adjoint_over_fwd"] = result["adjoint"]["ratio"]
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(prob, dt)
     print(json.dumps(result), flush=True)

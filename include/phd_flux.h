@@ -246,6 +246,35 @@ int phd_ghost_refresh_eos(phd_handle* h, phd_plan* p, const phd_cells* c,
                           int boundary, const double* lo, const double* hi,
                           double gamma);
 
+/* ---- reverse mode (adjoint) ----------------------------------------------
+ * No counterpart in the reference, which has no derivatives; these serve
+ * gradient-based inversion and optimisation through the step above.  The
+ * differentiated step is phd_core_step (AREPO limiter, HLLC, face
+ * velocities held fixed: a static or prescribed mesh) followed by
+ * phd_ghost_refresh_eos. */
+typedef struct phd_adjoint phd_adjoint;
+/* Per-mesh adjoint workspace (mirror-entry table, ghost lists, per-face and
+ * per-cell adjoint buffers).  Allocating and synchronising; recreate with
+ * the plan. */
+int phd_adjoint_create(phd_handle* h, phd_plan* p, const phd_cells* c,
+                       const phd_faces* f, phd_adjoint** out);
+int phd_adjoint_destroy(phd_adjoint* a);
+/* Given lam_out = dL/dcons after the step (real cells; dim+2 arrays of
+ * n_cells), writes lam_in = dL/dcons before it.  `c` holds the state the
+ * step started from (conserved and primitive fields consistent, ghosts
+ * refreshed); `rec` is a cell-record buffer (phd_cell_record_ld), rebuilt
+ * here.  Deterministic: every sum runs in a fixed order. */
+int phd_core_step_vjp(phd_handle* h, phd_plan* p, phd_adjoint* a, const phd_cells* c,
+                      const phd_faces* f, double* rec, int32_t rec_ld, int boundary,
+                      const double* lo, const double* hi, double gamma, double dt,
+                      int boost, int gizmo, double* const* lam_out,
+                      double* const* lam_in);
+/* lam_x += J^T lam_y for the equation of state of the real cells:
+ * to_prim = 1: y = prim(cons), x = cons;  to_prim = 0: y = cons(prim),
+ * x = prim (equation_state.pyx:43-117). */
+int phd_eos_vjp(phd_handle* h, const phd_cells* c, double gamma, int to_prim,
+                double* const* lam_y, double* const* lam_x);
+
 #ifdef __cplusplus
 }
 #endif

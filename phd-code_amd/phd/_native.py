@@ -92,6 +92,11 @@ _FLUX_SIGS = {
     "phd_core_step": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_dbl,
                        c_dbl, c_i32, c_i32, c_i32, c_vp, c_i32], c_i32),
     "phd_ghost_refresh_eos": ([c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_dbl], c_i32),
+    "phd_adjoint_create": ([c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)], c_i32),
+    "phd_adjoint_destroy": ([c_vp], c_i32),
+    "phd_core_step_vjp": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_dbl,
+                           c_dbl, c_i32, c_i32, c_vp, c_vp], c_i32),
+    "phd_eos_vjp": ([c_vp, c_vp, c_dbl, c_i32, c_vp, c_vp], c_i32),
     "phd_gradients_packed": ([c_vp, c_vp, c_vp, c_i32, c_vp, c_i32], c_i32),
     "phd_face_summands": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_dbl, c_dbl, c_i32, c_i32,
                            c_i32, c_vp], c_i32),

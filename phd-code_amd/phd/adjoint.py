@@ -1,0 +1,217 @@
+"""Reverse mode of the static-mesh hydro step, on the GPU.
+
+StaticStepper runs StaticMeshMUSCLHancock.evolve_timestep (integrate.py:
+349-378) on a frozen mesh through the C ABI: phd_core_step (gradients +
+AREPO limiter, ghost gradients, states + HLLC, update) and
+phd_ghost_refresh_eos.  Its vjp() is the hand-written reverse of one such
+step (libphd_flux.so: phd_core_step_vjp), deterministic like the forward.
+
+Rollout is a torch.autograd.Function over n steps: forward keeps a
+checkpoint of the full cell state every `every` steps; backward replays each
+segment from its checkpoint, keeping the segment's step states, and runs the
+step VJPs in reverse.  Gradients are taken with respect to the initial
+primitive fields of the real cells; the time steps are those the forward
+took (CFL on each step's state), held fixed.
+"""
+import ctypes
+
+import torch
+
+from . import _native as nat
+from .views import cells_view, faces_view
+
+
+class StaticStepper(object):
+    def __init__(self, problem):
+        self.p = problem
+        self.lib, self.h = nat.flux(), nat.handle()
+        pc, mesh = problem.particles, problem.mesh
+        dim = problem.dim
+        self.dim, self.nf = dim, dim + 2
+        # static mesh: generator and face velocities are zero (integrate.py:339-347)
+        for f in pc.carray_named_groups["w"]:
+            pc[f][:] = 0.0
+        for f in mesh.faces.carray_named_groups["velocity"]:
+            mesh.faces[f][:] = 0.0
+        self.cells, self.faces = cells_view(pc), faces_view(mesh)
+        recon = problem.recon
+        if recon.slope_limiter != nat.LIMITER_AREPO:
+            raise RuntimeError("the adjoint supports the AREPO limiter")
+        if problem.riemann.solver != nat.SOLVER_HLLC:
+            raise RuntimeError("the adjoint supports the HLLC solver")
+        recon.grad.resize(pc.get_carray_size())
+        self.grad = recon.grad
+        problem.riemann.fluxes.resize(mesh.faces.get_carray_size())
+        dm = problem.domain_manager
+        self.lo = (ctypes.c_double * 3)(*(dm.xmin + [0.0] * (3 - dim)))
+        self.hi = (ctypes.c_double * 3)(*(dm.xmax + [0.0] * (3 - dim)))
+        self.boundary = problem.boundary.native_code
+        self.gamma = problem.eos.get_gamma()
+        self.gizmo = int(recon.gizmo_limiter)
+        self.boost = int(problem.riemann.boost)
+        self.plan = mesh.plan(pc)
+        adj = ctypes.c_void_p()
+        nat.check(self.lib.phd_adjoint_create(self.h, self.plan, ctypes.byref(self.cells),
+                                              ctypes.byref(self.faces), ctypes.byref(adj)),
+                  "phd_adjoint_create")
+        self.adj = adj.value
+        pc_ = pc
+        self.real = pc_["tag"] == 0
+        self.cons_names = pc_.carray_named_groups["conservative"]
+        self.prim_names = pc_.carray_named_groups["primitive"]
+        self.capture_initial()
+
+    def capture_initial(self):
+        """Take the current real-cell primitives as the initial state that
+        rollouts start from (the fields they do not differentiate)."""
+        pc = self.p.particles
+        self.prim_init = [pc[f][self.real].clone() for f in self.prim_names]
+
+    def close(self):
+        if self.adj:
+            self.lib.phd_adjoint_destroy(self.adj)
+            self.adj = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state ------------------------------------------------------------------
+    def state(self):
+        pc = self.p.particles
+        return [pc[f].clone() for f in self.cons_names + self.prim_names]
+
+    def set_state(self, s):
+        pc = self.p.particles
+        for f, t in zip(self.cons_names + self.prim_names, s):
+            pc[f].copy_(t)
+
+    def refresh(self):
+        """Ghost conserved fields from the reals + EOS, all cells."""
+        nat.check(self.lib.phd_ghost_refresh_eos(self.h, self.plan, ctypes.byref(self.cells),
+                                                 self.boundary,
+                                                 ctypes.cast(self.lo, ctypes.c_void_p),
+                                                 ctypes.cast(self.hi, ctypes.c_void_p),
+                                                 self.gamma), "phd_ghost_refresh_eos")
+
+    def time_step(self):
+        return self.p.riemann.compute_time_step(self.p.particles, self.p.eos)
+
+    # -- one step and its reverse ----------------------------------------------
+    def step(self, dt):
+        rc = self.lib.phd_core_step(self.h, self.plan, ctypes.byref(self.cells),
+                                    ctypes.byref(self.faces), self.grad.data.data_ptr(),
+                                    self.grad.ld, nat.LIMITER_AREPO, self.boundary,
+                                    ctypes.cast(self.lo, ctypes.c_void_p),
+                                    ctypes.cast(self.hi, ctypes.c_void_p), self.gamma, float(dt),
+                                    self.boost, self.gizmo, nat.SOLVER_HLLC, None, 0)
+        nat.check(rc, "phd_core_step")
+        self.refresh()
+
+    def vjp(self, dt, lam_out):
+        """lam_out: dL/dcons after the step (dim+2 tensors over all cells,
+        real entries used); returns dL/dcons before it.  The cells must hold
+        the state the step started from."""
+        lam_in = [torch.zeros_like(t) for t in lam_out]
+        rc = self.lib.phd_core_step_vjp(self.h, self.plan, self.adj, ctypes.byref(self.cells),
+                                        ctypes.byref(self.faces), self.grad.data.data_ptr(),
+                                        self.grad.ld, self.boundary,
+                                        ctypes.cast(self.lo, ctypes.c_void_p),
+                                        ctypes.cast(self.hi, ctypes.c_void_p), self.gamma,
+                                        float(dt), self.boost, self.gizmo,
+                                        nat.ptr_array(lam_out), nat.ptr_array(lam_in))
+        nat.check(rc, "phd_core_step_vjp")
+        return lam_in
+
+    def eos_vjp(self, to_prim, lam_y):
+        lam_x = [torch.zeros_like(t) for t in lam_y]
+        nat.check(self.lib.phd_eos_vjp(self.h, ctypes.byref(self.cells), self.gamma,
+                                       int(to_prim), nat.ptr_array(lam_y),
+                                       nat.ptr_array(lam_x)), "phd_eos_vjp")
+        return lam_x
+
+    def set_initial(self, prim_real):
+        """Primitive fields of the real cells (dim+2 tensors of n_real),
+        conserved fields from them, ghosts refreshed."""
+        pc = self.p.particles
+        for f, t in zip(self.prim_names, prim_real):
+            pc[f][self.real] = t
+        self.p.eos.conservative_from_primitive(pc)
+        self.refresh()
+
+
+class Rollout(torch.autograd.Function):
+    """Final real-cell density after n static-mesh steps, as a function of
+    the initial real-cell density and one initial velocity component."""
+
+    @staticmethod
+    def forward(ctx, rho0, v0, stepper, n_steps, every, axis, dts=None):
+        st = stepper
+        pc = st.p.particles
+        prim0 = [t.clone() for t in st.prim_init]
+        prim0[0] = rho0.detach()
+        prim0[1 + axis] = v0.detach()
+        st.set_initial(prim0)
+        checkpoints, dts, dts_in = {}, [], dts
+        for n in range(n_steps):
+            if n % every == 0:
+                checkpoints[n] = st.state()
+            dt = st.time_step() if dts_in is None else dts_in[n]
+            dts.append(dt)
+            st.step(dt)
+        ctx.st, ctx.n, ctx.every, ctx.axis = st, n_steps, every, axis
+        ctx.checkpoints, ctx.dts, ctx.prim0 = checkpoints, dts, prim0
+        ctx.final = st.state()
+        return pc["density"][st.real].clone()
+
+    @staticmethod
+    def backward(ctx, g_rho):
+        st = ctx.st
+        pc = st.p.particles
+        nfld = st.nf
+        # final density = prim(cons_N)[0]
+        st.set_state(ctx.final)
+        lam_prim = [torch.zeros_like(pc[f]) for f in st.prim_names]
+        lam_prim[0][st.real] = g_rho
+        lam = st.eos_vjp(1, lam_prim)
+        starts = sorted(ctx.checkpoints)
+        for s0 in reversed(starts):
+            s1 = min(s0 + ctx.every, ctx.n)
+            st.set_state(ctx.checkpoints[s0])
+            states = []
+            for n in range(s0, s1):
+                states.append(st.state())
+                st.step(ctx.dts[n])
+            for n in reversed(range(s0, s1)):
+                st.set_state(states[n - s0])
+                lam = st.vjp(ctx.dts[n], lam)
+        # initial conserved fields = cons(prim_0)
+        st.set_state(ctx.checkpoints[0])
+        lam_p0 = st.eos_vjp(0, lam)
+        st.set_state(ctx.final)
+        return (lam_p0[0][st.real].clone(), lam_p0[1 + ctx.axis][st.real].clone(), None, None,
+                None, None, None)
+
+
+def rollout(stepper, rho0, v0, n_steps, every=10, axis=1, dts=None):
+    """Differentiable final density after n_steps (see Rollout).  dts: the
+    time steps to take (default: the CFL step of each state)."""
+    return Rollout.apply(rho0, v0, stepper, n_steps, every, axis, dts)
+
+
+def forward_dts(stepper, rho0, v0, n_steps, axis=1, dts=None):
+    """Plain forward run; returns (final real density, time steps taken)."""
+    st = stepper
+    pc = st.p.particles
+    prim0 = [t.clone() for t in st.prim_init]
+    prim0[0] = rho0
+    prim0[1 + axis] = v0
+    st.set_initial(prim0)
+    out = []
+    for n in range(n_steps):
+        dt = st.time_step() if dts is None else dts[n]
+        out.append(dt)
+        st.step(dt)
+    return pc["density"][st.real].clone(), out

@@ -337,8 +337,12 @@ PHD_HD inline void hllc_vjp(const double* L, const double* R, const double* n, c
     } else {
         side_flux_vjp<D>(R, n, Vnr, vr_sq, gamma, lF, lR, lVnr, lvr);
     }
-    get_waves_vjp(L[0], Vnl, L[D + 1], R[0], Vnr, R[D + 1], g, lsl, lsc, lsr, lL[0], lVnl,
-                  lL[D + 1], lR[0], lVnr, lR[D + 1]);
+    // the upwind side fluxes do not read the wave estimates; skipping their
+    // reverse also keeps a NaN estimate (vacuum-forming states, where the
+    // forward falls through to an upwind flux) out of the lambdas
+    if (lsl != 0. || lsc != 0. || lsr != 0.)
+        get_waves_vjp(L[0], Vnl, L[D + 1], R[0], Vnr, R[D + 1], g, lsl, lsc, lsr, lL[0], lVnl,
+                      lL[D + 1], lR[0], lVnr, lR[D + 1]);
     for (int k = 0; k < D; k++) {
         lL[1 + k] += lVnl * n[k] + 2. * lvl * L[1 + k];
         lR[1 + k] += lVnr * n[k] + 2. * lvr * R[1 + k];
