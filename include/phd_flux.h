@@ -259,16 +259,23 @@ typedef struct phd_adjoint phd_adjoint;
 int phd_adjoint_create(phd_handle* h, phd_plan* p, const phd_cells* c,
                        const phd_faces* f, phd_adjoint** out);
 int phd_adjoint_destroy(phd_adjoint* a);
+/* Flags of phd_core_step_vjp */
+#define PHD_VJP_REC_FRESH 1   /* `rec` already holds this state's limited and
+                                 ghost gradients (what phd_core_step wrote into
+                                 it from this state) and the plan's packed rows
+                                 are current (phd_ghost_refresh_eos ran with
+                                 this plan since the fields last changed) */
 /* Given lam_out = dL/dcons after the step (real cells; dim+2 arrays of
- * n_cells), writes lam_in = dL/dcons before it.  `c` holds the state the
- * step started from (conserved and primitive fields consistent, ghosts
- * refreshed); `rec` is a cell-record buffer (phd_cell_record_ld), rebuilt
- * here.  Deterministic: every sum runs in a fixed order. */
+ * n_cells), writes lam_in = dL/dcons before it (real entries; the ghost
+ * entries are not touched).  `c` holds the state the step started from
+ * (conserved and primitive fields consistent, ghosts refreshed); `rec` is a
+ * cell-record buffer (phd_cell_record_ld), rebuilt here unless flags has
+ * PHD_VJP_REC_FRESH.  Deterministic: every sum runs in a fixed order. */
 int phd_core_step_vjp(phd_handle* h, phd_plan* p, phd_adjoint* a, const phd_cells* c,
                       const phd_faces* f, double* rec, int32_t rec_ld, int boundary,
                       const double* lo, const double* hi, double gamma, double dt,
                       int boost, int gizmo, double* const* lam_out,
-                      double* const* lam_in);
+                      double* const* lam_in, int flags);
 /* lam_x += J^T lam_y for the equation of state of the real cells:
  * to_prim = 1: y = prim(cons), x = cons;  to_prim = 0: y = cons(prim),
  * x = prim (equation_state.pyx:43-117). */

@@ -31,6 +31,7 @@ LIMITER_AREPO, LIMITER_TESS = 0, 1
 BOUNDARY_REFLECTIVE, BOUNDARY_PERIODIC = 0, 1
 MODE_BITWISE = 0
 CORE_XP_FRESH = 1
+VJP_REC_FRESH = 1
 STAGE_SPATIAL, STAGE_TEMPORAL = 1, 2
 ORDER_CONSTANT, ORDER_LINEAR = 0, 1
 
@@ -95,7 +96,7 @@ _FLUX_SIGS = {
     "phd_adjoint_create": ([c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)], c_i32),
     "phd_adjoint_destroy": ([c_vp], c_i32),
     "phd_core_step_vjp": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_dbl,
-                           c_dbl, c_i32, c_i32, c_vp, c_vp], c_i32),
+                           c_dbl, c_i32, c_i32, c_vp, c_vp, c_i32], c_i32),
     "phd_eos_vjp": ([c_vp, c_vp, c_dbl, c_i32, c_vp, c_vp], c_i32),
     "phd_gradients_packed": ([c_vp, c_vp, c_vp, c_i32, c_vp, c_i32], c_i32),
     "phd_face_summands": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_dbl, c_dbl, c_i32, c_i32,

@@ -59,6 +59,7 @@ class StaticStepper(object):
         self.real = pc_["tag"] == 0
         self.cons_names = pc_.carray_named_groups["conservative"]
         self.prim_names = pc_.carray_named_groups["primitive"]
+        self.xp_of = None           # the cells view the plan's packed rows were built from
         self.capture_initial()
 
     def capture_initial(self):
@@ -87,41 +88,80 @@ class StaticStepper(object):
         pc = self.p.particles
         for f, t in zip(self.cons_names + self.prim_names, s):
             pc[f].copy_(t)
+        self.xp_of = None
 
-    def refresh(self):
-        """Ghost conserved fields from the reals + EOS, all cells."""
-        nat.check(self.lib.phd_ghost_refresh_eos(self.h, self.plan, ctypes.byref(self.cells),
+    def save_cons(self, dst):
+        """Copy the conserved fields (all cells) into `dst`."""
+        pc = self.p.particles
+        for t, f in zip(dst, self.cons_names):
+            t.copy_(pc[f])
+
+    def view(self, cons, prim):
+        """A cells view of the same mesh whose conserved and primitive
+        fields live in the given tensors (dim+2 each, all cells)."""
+        v = nat.Cells.from_buffer_copy(self.cells)
+        for q, t in enumerate(cons):
+            v.cons[q] = t.data_ptr()
+        for q, t in enumerate(prim):
+            v.prim[q] = t.data_ptr()
+        v._keep = (cons, prim)
+        return v
+
+    def refresh(self, cells=None):
+        """Ghost conserved fields from the reals + EOS, all cells of `cells`
+        (default: the container); also rewrites the plan's packed rows."""
+        c = self.cells if cells is None else cells
+        nat.check(self.lib.phd_ghost_refresh_eos(self.h, self.plan, ctypes.byref(c),
                                                  self.boundary,
                                                  ctypes.cast(self.lo, ctypes.c_void_p),
                                                  ctypes.cast(self.hi, ctypes.c_void_p),
                                                  self.gamma), "phd_ghost_refresh_eos")
+        self.xp_of = c
 
     def time_step(self):
         return self.p.riemann.compute_time_step(self.p.particles, self.p.eos)
 
     # -- one step and its reverse ----------------------------------------------
-    def step(self, dt):
+    def new_rec(self):
+        """A cell-record buffer shaped like the stepper's own."""
+        return torch.empty_like(self.grad.data)
+
+    def step(self, dt, rec=None):
+        """One step + refresh.  rec: a buffer (new_rec()) that keeps this
+        state's limited and ghost gradients for a later vjp(rec=...)."""
+        r = self.grad.data if rec is None else rec
+        flags = nat.CORE_XP_FRESH if self.xp_of is self.cells else 0
         rc = self.lib.phd_core_step(self.h, self.plan, ctypes.byref(self.cells),
-                                    ctypes.byref(self.faces), self.grad.data.data_ptr(),
+                                    ctypes.byref(self.faces), r.data_ptr(),
                                     self.grad.ld, nat.LIMITER_AREPO, self.boundary,
                                     ctypes.cast(self.lo, ctypes.c_void_p),
                                     ctypes.cast(self.hi, ctypes.c_void_p), self.gamma, float(dt),
-                                    self.boost, self.gizmo, nat.SOLVER_HLLC, None, 0)
+                                    self.boost, self.gizmo, nat.SOLVER_HLLC, None, flags)
         nat.check(rc, "phd_core_step")
         self.refresh()
 
-    def vjp(self, dt, lam_out):
+    def vjp(self, dt, lam_out, rec=None, out=None, cells=None):
         """lam_out: dL/dcons after the step (dim+2 tensors over all cells,
-        real entries used); returns dL/dcons before it.  The cells must hold
-        the state the step started from."""
-        lam_in = [torch.zeros_like(t) for t in lam_out]
-        rc = self.lib.phd_core_step_vjp(self.h, self.plan, self.adj, ctypes.byref(self.cells),
-                                        ctypes.byref(self.faces), self.grad.data.data_ptr(),
+        real entries used); returns dL/dcons before it (real entries written;
+        the ghost entries of `out` are left as they are, zeros when None).
+        `cells` (default: the container) must hold the state the step
+        started from; rec: the gradients step(rec=...) kept for this state,
+        after which the state must have been refreshed (refresh(cells))."""
+        c = self.cells if cells is None else cells
+        lam_in = [torch.zeros_like(t) for t in lam_out] if out is None else out
+        flags = 0
+        if rec is not None:
+            if self.xp_of is not c:
+                raise RuntimeError("vjp(rec=...) needs refresh() of these cells first")
+            flags = nat.VJP_REC_FRESH
+        r = self.grad.data if rec is None else rec
+        rc = self.lib.phd_core_step_vjp(self.h, self.plan, self.adj, ctypes.byref(c),
+                                        ctypes.byref(self.faces), r.data_ptr(),
                                         self.grad.ld, self.boundary,
                                         ctypes.cast(self.lo, ctypes.c_void_p),
                                         ctypes.cast(self.hi, ctypes.c_void_p), self.gamma,
                                         float(dt), self.boost, self.gizmo,
-                                        nat.ptr_array(lam_out), nat.ptr_array(lam_in))
+                                        nat.ptr_array(lam_out), nat.ptr_array(lam_in), flags)
         nat.check(rc, "phd_core_step_vjp")
         return lam_in
 
@@ -139,12 +179,20 @@ class StaticStepper(object):
         for f, t in zip(self.prim_names, prim_real):
             pc[f][self.real] = t
         self.p.eos.conservative_from_primitive(pc)
+        self.xp_of = None
         self.refresh()
 
 
 class Rollout(torch.autograd.Function):
     """Final real-cell density after n static-mesh steps, as a function of
-    the initial real-cell density and one initial velocity component."""
+    the initial real-cell density and one initial velocity component.
+
+    Backward, per segment [s0, s1): restore the checkpoint, replay steps
+    s0..s1-2 keeping each start state's conserved fields and gradient
+    records, run the VJP of step s1-1 on the state now in place, then for
+    each earlier step refresh a cells view over its kept conserved fields
+    (primitives, ghosts and packed rows are functions of them) and run its
+    VJP there on the kept gradients: no copies back."""
 
     @staticmethod
     def forward(ctx, rho0, v0, stepper, n_steps, every, axis, dts=None):
@@ -176,17 +224,27 @@ class Rollout(torch.autograd.Function):
         lam_prim = [torch.zeros_like(pc[f]) for f in st.prim_names]
         lam_prim[0][st.real] = g_rho
         lam = st.eos_vjp(1, lam_prim)
+        bufs = (lam, [torch.zeros_like(t) for t in lam])   # ghost entries stay zero
+        nseg = min(ctx.every, ctx.n)
+        snaps = [[torch.empty_like(pc[f]) for f in st.cons_names] for _ in range(nseg)]
+        prim_tmp = [torch.empty_like(pc[f]) for f in st.prim_names]
+        views = [st.view(sn, prim_tmp) for sn in snaps]
+        recs = [st.new_rec() for _ in range(nseg)]
         starts = sorted(ctx.checkpoints)
         for s0 in reversed(starts):
             s1 = min(s0 + ctx.every, ctx.n)
             st.set_state(ctx.checkpoints[s0])
-            states = []
-            for n in range(s0, s1):
-                states.append(st.state())
-                st.step(ctx.dts[n])
+            for n in range(s0, s1 - 1):
+                st.save_cons(snaps[n - s0])
+                st.step(ctx.dts[n], rec=recs[n - s0])
             for n in reversed(range(s0, s1)):
-                st.set_state(states[n - s0])
-                lam = st.vjp(ctx.dts[n], lam)
+                out = bufs[1] if lam is bufs[0] else bufs[0]
+                if n == s1 - 1:
+                    lam = st.vjp(ctx.dts[n], lam, out=out)
+                else:
+                    st.refresh(views[n - s0])
+                    lam = st.vjp(ctx.dts[n], lam, rec=recs[n - s0], out=out,
+                                 cells=views[n - s0])
         # initial conserved fields = cons(prim_0)
         st.set_state(ctx.checkpoints[0])
         lam_p0 = st.eos_vjp(0, lam)

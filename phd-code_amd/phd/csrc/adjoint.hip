@@ -3,24 +3,27 @@
 // The step being differentiated is what phd_core_step + phd_ghost_refresh_eos
 // do with face velocities held fixed (a static or prescribed mesh):
 //
-//   prim = EOS(cons) (ghosts: images, boundary flips)            a10, f1
-//   rec  = limited gradients (eq. 21 + AREPO), ghost gradients   a4, a5
-//   L, R = extrapolation + GIZMO, MUSCL-Hancock predictor at dt/2  a6, a7
-//   F    = HLLC + deboost                                         a8
-//   cons' = cons - sum (dt*area) F  over owned faces, + over incoming   a9
+//   prim = EOS(cons) (ghosts: images, boundary flips)
+//   rec  = limited gradients (eq. 21 + AREPO), ghost gradients
+//   L, R = extrapolation + GIZMO, MUSCL-Hancock predictor at dt/2
+//   F    = HLLC + deboost
+//   cons' = cons - sum (dt*area) F over owned faces, + over incoming
 //
 // phd_core_step_vjp maps lambda' = dL/dcons' (real cells) to
 // lambda = dL/dcons, all reductions deterministic:
-//   K1 face VJP     one thread per face: lambda of (p_i, g_i, p_j, g_j)
-//                   written as one row per face
-//   K2 cell gather  one thread per cell (ghosts too): sums its faces' rows
-//                   in ascending face id
-//   K3 ghost fold   one thread per real cell: adds its ghost copies'
-//                   lambdas with the boundary flips
-//   K4 gradient VJP one thread per real cell: lambda of its own and its
-//                   neighbours' primitives, one vector per list entry
-//   K5 combine      one thread per real cell: own + neighbour entries
-//                   (through the mirror-entry table), EOS VJP, + lambda'
+//   K1a face, Riemann   one thread per face: forward states, HLLC reverse;
+//                       lambda of the predicted states L, R into the row
+//   K1b face, states    one thread per face: predictor and extrapolation
+//                       reverse; lambda of (p_i, g_i, p_j, g_j) as the row
+//                       (two kernels: one would hold 272 registers, one
+//                       wave per SIMD)
+//   K2 cell             one thread per real cell: its face rows and its
+//                       ghost copies' (with the boundary flips), in list
+//                       order, then the gradient VJP: lambda of its own
+//                       primitives, and one vector per list entry for the
+//                       entry's other cell
+//   K3 combine          one thread per real cell: own + neighbour entries
+//                       (through the mirror-entry table), EOS VJP, + lambda'
 #include "flux_common.h"
 #include "adjoint_dev.h"
 
@@ -33,7 +36,6 @@ struct phd_adjoint {
     int32_t* gofs;          // n_real + 1: ghost copies of each real cell
     int32_t* gids;
     double* frow;           // F x FW face adjoint rows
-    double* lpg;            // Na x (NF+G): lambda of each cell's primitives | gradient
     double* contrib;        // n_entries x NF: gradient-VJP lambda for the entry's other cell
     double* lself;          // Na x NF: gradient-VJP lambda for the cell's own primitives
     void* base;
@@ -82,14 +84,58 @@ __device__ __forceinline__ int ghost_flips(const phd_cells& c, int g, int reflec
     return m;
 }
 
-// K1
+// K1a: forward face states + HLLC reverse -> lambda of the predicted states
 template <int D>
-__global__ __launch_bounds__(PHD_BLOCK) void k_adj_face(phd_cells c, phd_faces f,
-                                                        const double* __restrict__ frec,
-                                                        const double* __restrict__ rec, int ld,
-                                                        Gas gas, double dt, int boost, int gizmo,
-                                                        FaceArrays lam_out,
-                                                        double* __restrict__ frow)
+__global__ __launch_bounds__(PHD_BLOCK) void k_adj_face_a(phd_cells c, phd_faces f,
+                                                          const double* __restrict__ frec,
+                                                          const double* __restrict__ rec, int ld,
+                                                          Gas gas, double dt, int boost, int gizmo,
+                                                          FaceArrays lam_out,
+                                                          double* __restrict__ frow)
+{
+    using Y = Lay<D>;
+    constexpr int NF = Y::NF, W = frow_width<D>();
+    long long m = xcd_index(f.n_faces);
+    if (m >= f.n_faces) return;
+    const int i = f.pair_i[m], j = f.pair_j[m];
+    double com[D], n[D], area, diff, fv[D];
+    load_face_rec<D>(frec, m, com, area, n, diff);
+#pragma unroll
+    for (int k = 0; k < D; k++) fv[k] = f.velocity[k][m];
+    double ri[Y::LD], rj[Y::LD];
+    load_rec<D>(rec, ld, i, ri);
+    load_rec<D>(rec, ld, j, rj);
+    const double* pi = ri + Y::P0;
+    const double* pj = rj + Y::P0;
+    const double* gi = ri + Y::G0;
+    const double* gj = rj + Y::G0;
+    double L[NF], R[NF];
+    boost_copy<D>(pi, pj, fv, boost, L, R);
+    spatial_states<D>(com, diff, ri + Y::C0, rj + Y::C0, pi, pj, gi, gj, gizmo, L, R);
+    const double dt_half = 0.5 * dt;
+    temporal_side<D>(pi, gi, fv, boost, gas.gamma, dt_half, L);
+    temporal_side<D>(pj, gj, fv, boost, gas.gamma, dt_half, R);
+    const double t = dt * area;
+    const bool jreal = c.tag[j] == 0;
+    double lF[NF], lL[NF], lR[NF];
+#pragma unroll
+    for (int q = 0; q < NF; q++) {
+        lF[q] = t * ((jreal ? lam_out.q[q][j] : 0.0) - lam_out.q[q][i]);
+        lL[q] = lR[q] = 0.0;
+    }
+    hllc_vjp<D>(L, R, n, fv, gas, lF, lL, lR);
+    double* o = frow + (size_t)m * W;
+#pragma unroll
+    for (int q = 0; q < NF; q++) { o[q] = lL[q]; o[NF + q] = lR[q]; }
+}
+
+// K1b: predictor and extrapolation reverse -> the face row
+template <int D>
+__global__ __launch_bounds__(PHD_BLOCK) void k_adj_face_b(phd_cells c, phd_faces f,
+                                                          const double* __restrict__ frec,
+                                                          const double* __restrict__ rec, int ld,
+                                                          Gas gas, double dt, int boost, int gizmo,
+                                                          double* __restrict__ frow)
 {
     using Y = Lay<D>;
     constexpr int NF = Y::NF, G = Y::G, W = frow_width<D>();
@@ -107,28 +153,19 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_adj_face(phd_cells c, phd_faces f
     const double* pj = rj + Y::P0;
     const double* gi = ri + Y::G0;
     const double* gj = rj + Y::G0;
-    // forward: spatial states, then the predictor at dt/2
-    double Ls[NF], Rs[NF], L[NF], R[NF];
+    double Ls[NF], Rs[NF];
     boost_copy<D>(pi, pj, fv, boost, Ls, Rs);
     spatial_states<D>(com, diff, ri + Y::C0, rj + Y::C0, pi, pj, gi, gj, gizmo, Ls, Rs);
+    double lL[NF], lR[NF];
+    const double* r = frow + (size_t)m * W;
 #pragma unroll
-    for (int q = 0; q < NF; q++) { L[q] = Ls[q]; R[q] = Rs[q]; }
+    for (int q = 0; q < NF; q++) { lL[q] = r[q]; lR[q] = r[NF + q]; }
     const double dt_half = 0.5 * dt;
-    temporal_side<D>(pi, gi, fv, boost, gas.gamma, dt_half, L);
-    temporal_side<D>(pj, gj, fv, boost, gas.gamma, dt_half, R);
-    // lambda of the flux: cons_i -= (dt*area) F, cons_j += (dt*area) F
-    const double t = dt * area;
-    const bool jreal = c.tag[j] == 0;
-    double lF[NF];
+    double lLs[NF], lRs[NF], lpi[NF], lpj[NF], lgi[G], lgj[G];
 #pragma unroll
-    for (int q = 0; q < NF; q++) lF[q] = t * ((jreal ? lam_out.q[q][j] : 0.0) - lam_out.q[q][i]);
-    double lL[NF], lR[NF], lLs[NF], lRs[NF];
-    double lpi[NF], lpj[NF], lgi[G], lgj[G];
-#pragma unroll
-    for (int q = 0; q < NF; q++) { lL[q] = lR[q] = lLs[q] = lRs[q] = lpi[q] = lpj[q] = 0.0; }
+    for (int q = 0; q < NF; q++) { lLs[q] = lRs[q] = lpi[q] = lpj[q] = 0.0; }
 #pragma unroll
     for (int q = 0; q < G; q++) { lgi[q] = lgj[q] = 0.0; }
-    hllc_vjp<D>(L, R, n, fv, gas, lF, lL, lR);
     temporal_side_vjp<D>(pi, gi, fv, boost, gas.gamma, dt_half, Ls, lL, lLs, lpi, lgi);
     temporal_side_vjp<D>(pj, gj, fv, boost, gas.gamma, dt_half, Rs, lR, lRs, lpj, lgj);
     spatial_states_vjp<D>(com, diff, ri + Y::C0, rj + Y::C0, pi, pj, gi, gj, fv, boost, gizmo, lLs,
@@ -141,71 +178,6 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_adj_face(phd_cells c, phd_faces f
     double2* o = reinterpret_cast<double2*>(frow + (size_t)m * W);
 #pragma unroll
     for (int q = 0; q < W / 2; q++) o[q] = make_double2(row[2 * q], row[2 * q + 1]);
-}
-
-// K2
-template <int D>
-__global__ __launch_bounds__(PHD_BLOCK) void k_adj_gather(int n_cells, const int2* __restrict__ csr,
-                                                          const int32_t* __restrict__ off,
-                                                          const double* __restrict__ frow,
-                                                          double* __restrict__ lpg)
-{
-    constexpr int PW = pg_width<D>(), W = frow_width<D>();
-    long long cc = xcd_index(n_cells);
-    if (cc >= n_cells) return;
-    double s[PW];
-#pragma unroll
-    for (int q = 0; q < PW; q++) s[q] = 0.0;
-    for (int e = off[cc]; e < off[cc + 1]; e++) {
-        const int2 en = csr[e];
-        if ((en.y & 0x7fffffff) == INVALID) continue;
-        const double* r = frow + (size_t)en.x * W + ((en.y & INCOMING) ? PW : 0);
-#pragma unroll
-        for (int q = 0; q < PW; q++) s[q] += r[q];
-    }
-    double* o = lpg + (size_t)cc * PW;
-#pragma unroll
-    for (int q = 0; q < PW; q++) o[q] = s[q];
-}
-
-// K3: ghost g of real cell k has prim_g = flip(prim_k) (velocity q negated
-// for each flip axis q) and grad_g = grad_k with d v_q / d x_q negated
-template <int D>
-__global__ __launch_bounds__(PHD_BLOCK) void k_adj_ghost_fold(phd_cells c, int n_real,
-                                                              const int32_t* __restrict__ gofs,
-                                                              const int32_t* __restrict__ gids,
-                                                              int reflect, double lo0, double lo1,
-                                                              double lo2, double hi0, double hi1,
-                                                              double hi2, double* __restrict__ lpg)
-{
-    using Y = Lay<D>;
-    constexpr int NF = Y::NF, PW = pg_width<D>();
-    long long k = xcd_index(n_real);
-    if (k >= n_real) return;
-    const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
-    double s[PW];
-    double* o = lpg + (size_t)k * PW;
-#pragma unroll
-    for (int q = 0; q < PW; q++) s[q] = o[q];
-    for (int a = gofs[k]; a < gofs[k + 1]; a++) {
-        const int g = gids[a];
-        const int fl = ghost_flips<D>(c, g, reflect, lo, hi);
-        const double* r = lpg + (size_t)g * PW;
-        double t[PW];
-#pragma unroll
-        for (int q = 0; q < PW; q++) t[q] = r[q];
-#pragma unroll
-        for (int q = 0; q < D; q++) {
-            if (fl & (1 << q)) {
-                t[1 + q] = -t[1 + q];
-                t[NF + (1 + q) * D + q] = -t[NF + (1 + q) * D + q];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < PW; q++) s[q] += t[q];
-    }
-#pragma unroll
-    for (int q = 0; q < PW; q++) o[q] = s[q];
 }
 
 template <int D>
@@ -238,38 +210,84 @@ struct CsrNbr {
     }
 };
 
-// K4
+// sum of the face rows of cell cc's list, in list order (the cell's side of
+// each face row)
 template <int D>
-__global__ __launch_bounds__(PHD_BLOCK) void k_adj_grad(phd_cells c, const int2* __restrict__ csr,
+__device__ __forceinline__ void gather_rows(int cc, const int2* __restrict__ csr,
+                                            const int32_t* __restrict__ off,
+                                            const double* __restrict__ frow, double* s)
+{
+    constexpr int PW = pg_width<D>(), W = frow_width<D>();
+    for (int e = off[cc]; e < off[cc + 1]; e++) {
+        const int2 en = csr[e];
+        if ((en.y & 0x7fffffff) == INVALID) continue;
+        const double* r = frow + (size_t)en.x * W + ((en.y & INCOMING) ? PW : 0);
+#pragma unroll
+        for (int q = 0; q < PW; q++) s[q] += r[q];
+    }
+}
+
+// K2: one thread per real cell.  lambda of its primitives and limited
+// gradient = its face rows + those of its ghost copies (a ghost g of k has
+// prim_g = prim_k with velocity q negated for each flip axis q, and grad_g =
+// grad_k with d v_q / d x_q negated); then the gradient VJP: lself gets the
+// primitives' total (faces + own gradient), contrib one vector per list
+// entry for the entry's other cell
+template <int D>
+__global__ __launch_bounds__(PHD_BLOCK) void k_adj_cell(phd_cells c, int n_real,
+                                                        const int2* __restrict__ csr,
                                                         const int32_t* __restrict__ off,
-                                                        const double* __restrict__ xp,
+                                                        const int32_t* __restrict__ gofs,
+                                                        const int32_t* __restrict__ gids,
+                                                        int reflect, double lo0, double lo1,
+                                                        double lo2, double hi0, double hi1,
+                                                        double hi2, const double* __restrict__ xp,
                                                         const double* __restrict__ geo,
                                                         const double* __restrict__ frec,
-                                                        const double* __restrict__ lpg,
+                                                        const double* __restrict__ frow,
                                                         double* __restrict__ contrib,
                                                         double* __restrict__ lself)
 {
     using Y = Lay<D>;
     constexpr int NF = Y::NF, PW = pg_width<D>();
-    long long ii = xcd_index(c.n_cells);
-    if (ii >= c.n_cells) return;
+    long long ii = xcd_index(n_real);
+    if (ii >= n_real) return;
     const int i = (int)ii;
-    if (c.tag[i] != 0) return;
-    double pv[NF], xi[D], cx[D], vol, lg[Y::G], ls[NF];
+    const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+    double s[PW];
 #pragma unroll
-    for (int q = 0; q < NF; q++) { pv[q] = xp[(size_t)i * XPW + D + q]; ls[q] = 0.0; }
+    for (int q = 0; q < PW; q++) s[q] = 0.0;
+    gather_rows<D>(i, csr, off, frow, s);
+    for (int a = gofs[i]; a < gofs[i + 1]; a++) {
+        const int g = gids[a];
+        const int fl = ghost_flips<D>(c, g, reflect, lo, hi);
+        double t[PW];
+#pragma unroll
+        for (int q = 0; q < PW; q++) t[q] = 0.0;
+        gather_rows<D>(g, csr, off, frow, t);
+#pragma unroll
+        for (int q = 0; q < D; q++) {
+            if (fl & (1 << q)) {
+                t[1 + q] = -t[1 + q];
+                t[NF + (1 + q) * D + q] = -t[NF + (1 + q) * D + q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < PW; q++) s[q] += t[q];
+    }
+    double pv[NF], xi[D], cx[D], vol;
+#pragma unroll
+    for (int q = 0; q < NF; q++) pv[q] = xp[(size_t)i * XPW + D + q];
 #pragma unroll
     for (int k = 0; k < D; k++) { xi[k] = xp[(size_t)i * XPW + k]; cx[k] = geo[(size_t)i * GEO + k]; }
     vol = geo[(size_t)i * GEO + 3];
-#pragma unroll
-    for (int q = 0; q < Y::G; q++) lg[q] = lpg[(size_t)i * PW + NF + q];
     CsrNbr<D> nb{csr, off[i], off[i + 1] - off[i], xp, frec, contrib};
-    grad_cell_vjp<D>(nb, pv, xi, cx, vol, lg, ls);
+    grad_cell_vjp<D>(nb, pv, xi, cx, vol, s + NF, s);
 #pragma unroll
-    for (int q = 0; q < NF; q++) lself[(size_t)i * NF + q] = ls[q];
+    for (int q = 0; q < NF; q++) lself[(size_t)i * NF + q] = s[q];
 }
 
-// K5
+// K3
 template <int D>
 __global__ __launch_bounds__(PHD_BLOCK) void k_adj_combine(phd_cells c, int n_real,
                                                            const int2* __restrict__ csr,
@@ -280,7 +298,6 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_adj_combine(phd_cells c, int n_re
                                                            int reflect, double lo0, double lo1,
                                                            double lo2, double hi0, double hi1,
                                                            double hi2,
-                                                           const double* __restrict__ lpg,
                                                            const double* __restrict__ lself,
                                                            const double* __restrict__ contrib,
                                                            double gamma, FaceArrays lam_out,
@@ -293,7 +310,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_adj_combine(phd_cells c, int n_re
     const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
     double lp[NF];
 #pragma unroll
-    for (int q = 0; q < NF; q++) lp[q] = lpg[(size_t)k * PW + q] + lself[(size_t)k * NF + q];
+    for (int q = 0; q < NF; q++) lp[q] = lself[(size_t)k * NF + q];
     // neighbours' gradient VJPs that read this cell's primitives
     for (int e = off[k]; e < off[k + 1]; e++) {
         const int o = csr[e].y & 0x7fffffff;
@@ -365,29 +382,33 @@ FaceArrays fa5(double* const* a, int n)
 template <int D>
 int step_vjp(phd_handle* h, phd_plan* p, phd_adjoint* a, const phd_cells* c, const phd_faces* f,
              double* rec, int ld, int boundary, const double* lo, const double* hi, double gamma,
-             double dt, int boost, int gizmo, double* const* lam_out, double* const* lam_in)
+             double dt, int boost, int gizmo, double* const* lam_out, double* const* lam_in,
+             int flags)
 {
     double l[3] = {0, 0, 0}, u[3] = {0, 0, 0};
     const int refl = boundary == PHD_BOUNDARY_REFLECTIVE;
     if (refl) for (int k = 0; k < D; k++) { l[k] = lo[k]; u[k] = hi[k]; }
     // forward records of this step: packed rows, gradients, ghost gradients
-    int rc = phd_pack_cells(h, p, c);
-    if (!rc) rc = phd_gradients_packed(h, p, c, PHD_LIMITER_AREPO, rec, ld);
-    if (!rc) rc = phd_update_ghost_gradients(h, p, c, boundary, lo, hi, rec, ld);
-    if (rc) return rc;
+    if (!(flags & PHD_VJP_REC_FRESH)) {
+        int rc = phd_pack_cells(h, p, c);
+        if (!rc) rc = phd_gradients_packed(h, p, c, PHD_LIMITER_AREPO, rec, ld);
+        if (!rc) rc = phd_update_ghost_gradients(h, p, c, boundary, lo, hi, rec, ld);
+        if (rc) return rc;
+    }
     const FaceArrays Lo = fa5(lam_out, D + 2), Li = fa5(lam_in, D + 2);
-    if (f->n_faces > 0)
-        k_adj_face<D><<<xcd_grid(f->n_faces), PHD_BLOCK, 0, h->stream>>>(
-            *c, *f, p->face_rec, rec, ld, make_gas(gamma), dt, boost, gizmo, Lo, a->frow);
-    k_adj_gather<D><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(c->n_cells, p->csr, p->off,
-                                                                       a->frow, a->lpg);
-    k_adj_ghost_fold<D><<<xcd_grid(a->n_real), PHD_BLOCK, 0, h->stream>>>(
-        *c, a->n_real, a->gofs, a->gids, refl, l[0], l[1], l[2], u[0], u[1], u[2], a->lpg);
-    k_adj_grad<D><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(
-        *c, p->csr, p->off, p->xp, p->cell_geo, p->face_rec, a->lpg, a->contrib, a->lself);
+    if (f->n_faces > 0) {
+        const Gas gas = make_gas(gamma);
+        k_adj_face_a<D><<<xcd_grid(f->n_faces), PHD_BLOCK, 0, h->stream>>>(
+            *c, *f, p->face_rec, rec, ld, gas, dt, boost, gizmo, Lo, a->frow);
+        k_adj_face_b<D><<<xcd_grid(f->n_faces), PHD_BLOCK, 0, h->stream>>>(
+            *c, *f, p->face_rec, rec, ld, gas, dt, boost, gizmo, a->frow);
+    }
+    k_adj_cell<D><<<xcd_grid(a->n_real), PHD_BLOCK, 0, h->stream>>>(
+        *c, a->n_real, p->csr, p->off, a->gofs, a->gids, refl, l[0], l[1], l[2], u[0], u[1], u[2],
+        p->xp, p->cell_geo, p->face_rec, a->frow, a->contrib, a->lself);
     k_adj_combine<D><<<xcd_grid(a->n_real), PHD_BLOCK, 0, h->stream>>>(
         *c, a->n_real, p->csr, p->off, a->mirror, a->gofs, a->gids, refl, l[0], l[1], l[2], u[0],
-        u[1], u[2], a->lpg, a->lself, a->contrib, gamma, Lo, Li);
+        u[1], u[2], a->lself, a->contrib, gamma, Lo, Li);
     ADJ_CHK();
     return 0;
 }
@@ -429,7 +450,6 @@ int phd_adjoint_create(phd_handle* h, phd_plan* p, const phd_cells* c, const phd
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t s_m = al(4 * (size_t)ne), s_go = al(4 * (size_t)(nr + 1)), s_gi = al(4 * gids.size() + 4);
     const size_t s_fr = al(8 * (size_t)(2 * (NF + G)) * (size_t)nf);
-    const size_t s_pg = al(8 * (size_t)(NF + G) * (size_t)na);
     const size_t s_ct = al(8 * (size_t)NF * (size_t)ne), s_ls = al(8 * (size_t)NF * (size_t)na);
     phd_adjoint* a = new phd_adjoint();
     a->dim = D;
@@ -437,14 +457,13 @@ int phd_adjoint_create(phd_handle* h, phd_plan* p, const phd_cells* c, const phd
     a->n_faces = (int)nf;
     a->n_real = (int)nr;
     a->n_entries = ne;
-    a->bytes = s_m + s_go + s_gi + s_fr + s_pg + s_ct + s_ls;
+    a->bytes = s_m + s_go + s_gi + s_fr + s_ct + s_ls;
     if (hipMalloc(&a->base, a->bytes) != hipSuccess) { delete a; return PHD_E_HIP; }
     char* b = (char*)a->base;
     a->mirror = (int32_t*)b; b += s_m;
     a->gofs = (int32_t*)b; b += s_go;
     a->gids = (int32_t*)b; b += s_gi;
     a->frow = (double*)b; b += s_fr;
-    a->lpg = (double*)b; b += s_pg;
     a->contrib = (double*)b; b += s_ct;
     a->lself = (double*)b;
     bool ok = hipMemcpyAsync(a->gofs, gofs.data(), 4 * gofs.size(), hipMemcpyHostToDevice, h->stream) == hipSuccess;
@@ -476,15 +495,16 @@ int phd_adjoint_destroy(phd_adjoint* a)
 int phd_core_step_vjp(phd_handle* h, phd_plan* p, phd_adjoint* a, const phd_cells* c,
                       const phd_faces* f, double* rec, int32_t rec_ld, int boundary,
                       const double* lo, const double* hi, double gamma, double dt, int boost,
-                      int gizmo, double* const* lam_out, double* const* lam_in)
+                      int gizmo, double* const* lam_out, double* const* lam_in, int flags)
 {
     if (!h || !p || !a || !c || !f || !rec || !lam_out || !lam_in) return PHD_E_ARG;
+    if (flags & ~PHD_VJP_REC_FRESH) return PHD_E_ARG;
     if (a->n_cells != c->n_cells || a->n_faces != f->n_faces || a->dim != c->dim) return PHD_E_ARG;
     if (boundary != PHD_BOUNDARY_REFLECTIVE && boundary != PHD_BOUNDARY_PERIODIC) return PHD_E_MODE;
     if (boundary == PHD_BOUNDARY_REFLECTIVE && (!lo || !hi)) return PHD_E_ARG;
     if (rec_ld < phd_cell_record_ld(c->dim) || (rec_ld & 1) || (((uintptr_t)rec) & 15)) return PHD_E_ARG;
-    return c->dim == 2 ? step_vjp<2>(h, p, a, c, f, rec, rec_ld, boundary, lo, hi, gamma, dt, boost, gizmo, lam_out, lam_in)
-                       : step_vjp<3>(h, p, a, c, f, rec, rec_ld, boundary, lo, hi, gamma, dt, boost, gizmo, lam_out, lam_in);
+    return c->dim == 2 ? step_vjp<2>(h, p, a, c, f, rec, rec_ld, boundary, lo, hi, gamma, dt, boost, gizmo, lam_out, lam_in, flags)
+                       : step_vjp<3>(h, p, a, c, f, rec, rec_ld, boundary, lo, hi, gamma, dt, boost, gizmo, lam_out, lam_in, flags);
 }
 
 int phd_eos_vjp(phd_handle* h, const phd_cells* c, double gamma, int to_prim,
