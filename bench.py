@@ -207,6 +207,89 @@ def adjoint_timing(n=256, n_steps=50, every=10, reps=3):
             "ratio": round(bwd / fwd, 3)}
 
 
+def ensemble_timing(n_problems=4096, steps=100, per_problem_steps=3, seed=0, log=None):
+    """Batched ensemble of independent 2D Riemann problems (cells per problem
+    log-uniform in [1k, 64k]): autotuned schedule, one graph launch per step,
+    timed over `steps` steps; the per-problem form (each problem through
+    phd_core_step + phd_ghost_refresh_eos on its own plan, with the batched
+    run's per-problem time steps, so no host synchronisation) timed over
+    `per_problem_steps` steps; one step of both compared bit for bit."""
+    import torch
+    from phd import ensemble as E
+    log = log or (lambda m: None)
+    t0 = time.time()
+    spec = E.draw_problems(n_problems, seed=seed)
+    H = E.HostEnsemble(spec, n_workers=16)
+    log("ensemble host build %.1fs: %d problems, %d cells, %d faces" %
+        (time.time() - t0, n_problems, H.n_cells, H.n_faces))
+    t1 = time.time()
+    ens = E.Ensemble(H, hist_steps=steps + 8)
+    del H.x, H.dcom, H.area, H.com, H.normal
+    ens.reset()
+    torch.cuda.synchronize()
+    log("ensemble device set-up %.1fs" % (time.time() - t1))
+    best, tried = E.autotune(ens, steps=3, log=log)
+    s0 = ens.state()
+    # timed batched run
+    ens.restart(s0)
+    ens.run(2)                                  # warm
+    ens.restart(s0)
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    ens.run(steps)
+    torch.cuda.synchronize()
+    wall_b = time.perf_counter() - tb
+    _, k, hist, faults = ens.read(history=True)
+    ens.clear_errors()            # frozen problems' failed updates land there too
+    assert k == steps
+    # parity of one step at full size: batched vs per problem
+    t2 = time.time()
+    ens.restart(s0)
+    ens.run(1)
+    torch.cuda.synchronize()
+    after_b = ens.state()
+    ens.set_state(s0)
+    views = ens.problem_views()
+    log("per-problem plans %.1fs" % (time.time() - t2))
+    ens.step_problems(hist[0], first=True)
+    torch.cuda.synchronize()
+    same = all(torch.equal(a, b) for a, b in zip(after_b, ens.state()))
+    del after_b
+    # timed per-problem run (steps 1.. of the history)
+    ens.set_state(s0)
+    ens.step_problems(hist[0], first=True)
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    for s in range(1, 1 + per_problem_steps):
+        ens.step_problems(hist[s])
+    torch.cuda.synchronize()
+    wall_p = time.perf_counter() - tp
+    ens.clear_errors()
+    nf = ens.n_faces
+    ms_b = wall_b * 1e3 / steps
+    ms_p = wall_p * 1e3 / per_problem_steps
+    out = {"workload": "%d independent 2D Riemann problems (static jittered-lattice Voronoi "
+                       "meshes, cells log-uniform in [1k, 64k], reflective unit box, HLLC + "
+                       "PieceWiseLinear(arepo, gizmo), CFL 0.3 per problem), FP64" % n_problems,
+           "cells": ens.n_cells, "faces": nf, "steps": steps,
+           "batched_ms_per_step": round(ms_b, 3),
+           "batched_face_updates_per_s": nf / (ms_b * 1e-3),
+           "per_problem_ms_per_step": round(ms_p, 3),
+           "per_problem_steps_timed": per_problem_steps,
+           "speedup": round(ms_p / ms_b, 2),
+           "schedule": dict(zip(("grad_block", "face_block", "update_block", "xcd"), best)),
+           "autotune_ms": {"%d/%d/%d/%d" % c: round(v, 3) for c, v in sorted(tried.items())},
+           "bitwise_equal_to_per_problem": bool(same),
+           "frozen_problems": int((faults > 0).sum()),
+           "frozen_note": "a problem whose update left a non-positive mass/energy/density/"
+                          "pressure is frozen from the next step (the reference raises for "
+                          "its single problem); the others go on",
+           "launches_per_step": 1}
+    views = None
+    ens.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +298,9 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--workload", choices=("sedov", "ensemble"), default="sedov",
+                    help="ensemble: only the batched-ensemble measurement, as its own line")
+    ap.add_argument("--problems", type=int, default=4096)
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write the traffic record from two rocprofv3 --pmc runs of this "
                          "benchmark (FETCH_SIZE, WRITE_SIZE) and exit; no GPU needed")
@@ -237,6 +323,16 @@ def main():
     from phd import _native as nat
 
     torch.cuda.set_device(0)
+    if args.workload == "ensemble":
+        def log(m):
+            print(m, file=sys.stderr, flush=True)
+        r = ensemble_timing(args.problems, steps=args.steps, log=log)
+        r.update({"metric": "batched ensemble face-updates/s FP64 (one graph launch per step) "
+                            "and speed-up over per-problem launches",
+                  "value": r["batched_face_updates_per_s"], "unit": "face-updates/s",
+                  "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic"})
+        print(json.dumps(r), flush=True)
+        return
     t0 = time.time()
     prob = problems.sedov_3d(n=args.n, seed=0)
     step = problems.FrozenStep(prob)

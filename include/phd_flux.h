@@ -73,7 +73,9 @@ extern "C" {
 typedef struct phd_handle phd_handle;
 typedef struct phd_plan phd_plan;
 
-/* Cell (particle) arrays, n_cells = real + ghost.  Ghosts follow the reals. */
+/* Cell (particle) arrays, n_cells = real + ghost, told apart by tag.  The
+ * reference's containers put the ghosts after the reals, which the adjoint
+ * requires; a batched union puts each problem's ghosts after its reals. */
 typedef struct phd_cells {
     int32_t        dim;        /* 2 or 3 */
     int32_t        n_cells;    /* real + ghost */
@@ -245,6 +247,79 @@ int phd_core_step(phd_handle* h, phd_plan* p, const phd_cells* c,
 int phd_ghost_refresh_eos(phd_handle* h, phd_plan* p, const phd_cells* c,
                           int boundary, const double* lo, const double* hi,
                           double gamma);
+
+/* ---- batched ensembles ---------------------------------------------------
+ * No counterpart in the reference, which runs one problem per process.
+ * Many independent problems are laid out as one union mesh: problem p owns
+ * cells [cell_start[p], cell_start[p+1]) (its reals, then its ghosts) and a
+ * contiguous run of faces in owner order; face, neighbour-list and ghost-map
+ * indices are global, and no face joins two problems.  All problems share
+ * the boundary kind and box.  Every kernel of a step is flat over the union,
+ * so one launch serves all problems; the time step is per problem (a
+ * per-problem CFL minimum folded on the device).  A step can be captured
+ * once in a HIP graph and replayed with one graph launch per step, with no
+ * host synchronisation.  Static meshes (face velocities taken as zero, as
+ * StaticMeshMUSCLHancock sets them, integrate.py:340-347), limiter AREPO and
+ * solver HLLC only. */
+typedef struct {
+    int32_t grad_block;    /* threads per block of the gradient kernel: 64, 128, 256 */
+    int32_t face_block;    /* ... of the face kernel */
+    int32_t update_block;  /* ... of the update kernel */
+    int32_t xcd;           /* 1: blocks remapped so each XCD takes a contiguous
+                              eighth of the cells / faces; 0: dispatch order */
+} phd_schedule;
+typedef struct phd_batch phd_batch;
+/* cell_start: host array of n_problems + 1 offsets; hist_steps: capacity of
+ * the per-step time-step history (steps x problems, 0 for none).
+ * Allocating and synchronising; fails with PHD_E_ARG if a face joins two
+ * problems. */
+int phd_batch_create(phd_handle* h, phd_plan* p, const phd_cells* c,
+                     const phd_faces* f, int32_t n_problems,
+                     const int32_t* cell_start, int64_t hist_steps,
+                     phd_batch** out);
+int phd_batch_destroy(phd_batch* b);
+/* Launch schedule of the step's three main kernels (drops a captured
+ * graph).  Default {256, 256, 256, 1}. */
+int phd_batch_set_schedule(phd_batch* b, const phd_schedule* s);
+/* Ghost refresh + EOS of phd_ghost_refresh_eos over the union, also folding
+ * each problem's CFL minimum and fault flag for the next step.  Call after
+ * any change of the conserved fields made outside the batch's steps. */
+int phd_batch_refresh(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c,
+                      int boundary, const double* lo, const double* hi, double gamma,
+                      int boost);
+/* One step of every problem on h's stream, no host synchronisation:
+ * dt[p] = cfl * min over p's real cells of the CFL bound (riemann.pyx:
+ * 120-188) as the last refresh folded it, the core step of phd_core_step
+ * with that dt per problem, the batch refresh, and the history record.
+ * flags: PHD_CORE_XP_FRESH as for phd_core_step. */
+int phd_batch_step(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c,
+                   const phd_faces* f, double* rec, int32_t rec_ld, int boundary,
+                   const double* lo, const double* hi, double gamma, double cfl,
+                   int boost, int gizmo, int flags);
+/* Capture one phd_batch_step (packed rows fresh, as every step leaves them)
+ * into a graph on the batch's own stream.  The arrays named by the
+ * arguments must stay allocated while the graph is used. */
+int phd_batch_capture(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c,
+                      const phd_faces* f, double* rec, int32_t rec_ld, int boundary,
+                      const double* lo, const double* hi, double gamma, double cfl,
+                      int boost, int gizmo);
+/* n_steps graph launches, ordered after the work already on h's stream and
+ * before the work enqueued there afterwards; no host synchronisation.  The
+ * cells must hold a refreshed state (packed rows fresh). */
+int phd_batch_launch(phd_handle* h, phd_batch* b, int32_t n_steps);
+/* Reset the step counter of the history and the fault marks (on h's
+ * stream). */
+int phd_batch_reset_history(phd_handle* h, phd_batch* b);
+/* Synchronise and copy out: dt (n_problems, the last step's), the number of
+ * steps recorded since the last reset, the history (steps x problems) and
+ * the fault marks (n_problems: 0, or 1 + the step count at which the
+ * problem was frozen); NULL skips an output.  A problem is frozen (dt = 0,
+ * state unchanged) from the step after one that left a real cell with
+ * non-positive mass, energy, density or pressure, where the reference
+ * raises for its single problem; the other problems go on, and the fault is
+ * also recorded as for phd_core_step. */
+int phd_batch_read(phd_handle* h, phd_batch* b, double* dt, int64_t* steps,
+                   double* history, int32_t* faults);
 
 /* ---- reverse mode (adjoint) ----------------------------------------------
  * No counterpart in the reference, which has no derivatives; these serve

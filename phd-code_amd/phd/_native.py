@@ -55,6 +55,11 @@ class Cells(ctypes.Structure):
                 ("prim", c_vp * 5), ("cons", c_vp * 5), ("w", c_vp * 3)]
 
 
+class Schedule(ctypes.Structure):
+    _fields_ = [("grad_block", c_i32), ("face_block", c_i32), ("update_block", c_i32),
+                ("xcd", c_i32)]
+
+
 class Faces(ctypes.Structure):
     _fields_ = [("n_faces", c_i32), ("area", c_vp), ("com", c_vp * 3), ("normal", c_vp * 3),
                 ("velocity", c_vp * 3), ("pair_i", c_vp), ("pair_j", c_vp),
@@ -98,6 +103,18 @@ _FLUX_SIGS = {
     "phd_core_step_vjp": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_dbl,
                            c_dbl, c_i32, c_i32, c_vp, c_vp, c_i32], c_i32),
     "phd_eos_vjp": ([c_vp, c_vp, c_dbl, c_i32, c_vp, c_vp], c_i32),
+    "phd_batch_create": ([c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i64, ctypes.POINTER(c_vp)],
+                         c_i32),
+    "phd_batch_destroy": ([c_vp], c_i32),
+    "phd_batch_set_schedule": ([c_vp, c_vp], c_i32),
+    "phd_batch_step": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_dbl, c_dbl,
+                        c_i32, c_i32, c_i32], c_i32),
+    "phd_batch_capture": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_dbl,
+                           c_dbl, c_i32, c_i32], c_i32),
+    "phd_batch_launch": ([c_vp, c_vp, c_i32], c_i32),
+    "phd_batch_refresh": ([c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_dbl, c_i32], c_i32),
+    "phd_batch_reset_history": ([c_vp, c_vp], c_i32),
+    "phd_batch_read": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_i32),
     "phd_gradients_packed": ([c_vp, c_vp, c_vp, c_i32, c_vp, c_i32], c_i32),
     "phd_face_summands": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_dbl, c_dbl, c_i32, c_i32,
                            c_i32, c_vp], c_i32),

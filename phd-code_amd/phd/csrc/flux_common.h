@@ -79,15 +79,29 @@ inline unsigned nblocks(long long n) { return (unsigned)((n + PHD_BLOCK - 1) / P
 // grid for an XCD-remapped launch over n items: a multiple of 8 blocks
 inline unsigned xcd_grid(long long n) { unsigned nb = nblocks(n); return 8u * ((nb + 7u) / 8u); }
 
-// logical item index of this thread under the XCD remap; n items in total
-__device__ __forceinline__ long long xcd_index(long long n)
+// logical item index of this thread under a launch schedule: BS items per
+// block; XCD = 1 remaps blocks so that each of the 8 XCDs (which take
+// dispatched blocks round-robin) works on one contiguous eighth of the items
+// and keeps its neighbours' rows in its own L2.  n items in total.
+template <int BS, int XCD>
+__device__ __forceinline__ long long sched_index(long long n)
 {
-    const long long nb = (n + PHD_BLOCK - 1) / PHD_BLOCK;
-    const long long per = (nb + 7) / 8;
     const long long b = blockIdx.x;
+    if (!XCD) return b * BS + threadIdx.x;
+    const long long nb = (n + BS - 1) / BS;
+    const long long per = (nb + 7) / 8;
     const long long L = (b & 7) * per + (b >> 3);
-    return L * PHD_BLOCK + threadIdx.x;
+    return L * BS + threadIdx.x;
 }
+
+template <int BS, int XCD>
+inline unsigned sched_grid(long long n)
+{
+    const unsigned nb = (unsigned)((n + BS - 1) / BS);
+    return XCD ? 8u * ((nb + 7u) / 8u) : nb;
+}
+
+__device__ __forceinline__ long long xcd_index(long long n) { return sched_index<PHD_BLOCK, 1>(n); }
 
 __device__ __forceinline__ void ld2(const double* p, double& a, double& b)
 {

@@ -101,26 +101,34 @@ __device__ __forceinline__ double cell_radius(double vol)
 // CFL time step (riemann.pyx:120-188).  min is order independent: a
 // wavefront min and an atomic min on the bit pattern (non-negative doubles
 // order as unsigned integers) give the exact, deterministic minimum.
+// cfl_value: the bound of one real cell; +inf for NaN, which fmin would
+// otherwise skip
+template <int D>
+__device__ __forceinline__ double cfl_value(const phd_cells& c, long long i, double gamma, int boost)
+{
+    double v;
+    double d = c.prim[0][i], p = c.prim[D + 1][i];
+    double cs = sqrt(gamma * p / d);
+    double R = cell_radius<D>(c.volume[i]);
+    if (boost) {
+        v = R / cs;
+    } else {
+        double vsq = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) { double u = c.prim[1 + k][i]; vsq += u * u; }
+        v = R / (cs + sqrt(vsq));
+    }
+    if (!(v >= 0.0)) v = __longlong_as_double((long long)INF_BITS);
+    return v;
+}
+
 template <int D>
 __global__ __launch_bounds__(PHD_BLOCK) void k_cfl(phd_cells c, double gamma, int boost,
                                                    unsigned long long* err)
 {
     long long i = (long long)blockIdx.x * PHD_BLOCK + threadIdx.x;
     double v = __longlong_as_double((long long)INF_BITS);
-    if (i < c.n_cells && c.tag[i] == 0) {
-        double d = c.prim[0][i], p = c.prim[D + 1][i];
-        double cs = sqrt(gamma * p / d);
-        double R = cell_radius<D>(c.volume[i]);
-        if (boost) {
-            v = R / cs;
-        } else {
-            double vsq = 0.0;
-#pragma unroll
-            for (int k = 0; k < D; k++) { double u = c.prim[1 + k][i]; vsq += u * u; }
-            v = R / (cs + sqrt(vsq));
-        }
-        if (!(v >= 0.0)) v = __longlong_as_double((long long)INF_BITS);  // fmin skips NaN
-    }
+    if (i < c.n_cells && c.tag[i] == 0) v = cfl_value<D>(c, i, gamma, boost);
     for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
     __shared__ double wmin[PHD_BLOCK / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -417,9 +425,9 @@ __device__ __forceinline__ SlotEnt<D> load_slot(const int32_t* sj, const double*
 {
     SlotEnt<D> s;
     s.j = __builtin_nontemporal_load(sj + slot) & 0x7fffffff;
-    s.com[0] = __builtin_nontemporal_load(sx + slot);
-    s.com[1] = __builtin_nontemporal_load(sy + slot);
-    s.com[2] = D == 3 ? __builtin_nontemporal_load(sz + slot) : 0.0;
+    s.com[0] = sx[slot];               // cached: the limiter pass reads them again
+    s.com[1] = sy[slot];
+    s.com[2] = D == 3 ? sz[slot] : 0.0;
     s.area = __builtin_nontemporal_load(sa + slot);
     return s;
 }
@@ -444,8 +452,8 @@ __device__ __forceinline__ void load_row(const double* xp, int j, double* r)
 // correctly rounded quotient is monotone in its divisor, so the minimum over
 // faces is reached at the largest positive dphi and at the most negative
 // dphi: two divisions per field instead of one per face, same bits.
-template <int D, int LIM>
-__global__ __launch_bounds__(PHD_BLOCK) void k_grad_sell(int n_cells, const int8_t* __restrict__ tag,
+template <int D, int LIM, int BS = PHD_BLOCK, int XCD = 1>
+__global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __restrict__ tag,
                                                          const double* __restrict__ geo,
                                                          const double* __restrict__ xp,
                                                          const int32_t* __restrict__ off,
@@ -460,7 +468,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_grad_sell(int n_cells, const int8
 {
     using Y = Lay<D>;
     constexpr int NF = Y::NF;
-    const long long t = xcd_index(n_cells);
+    const long long t = sched_index<BS, XCD>(n_cells);
     if (t >= n_cells) return;
     const int i = perm[t];
     if (tag[i] != 0) return;
@@ -694,23 +702,39 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_riemann(phd_faces f, FaceArrays l
 // OUT bit 1: write the flux arrays (riemann.fluxes); bit 2: write
 // g[m] = (dt_full*area)*flux, the update's summand (mesh.pyx:530-544 forms
 // dt*a*flux for each endpoint; the product is the same for both).
-template <int D, int ORDER, int SOLVER, int OUT>
-__global__ __launch_bounds__(PHD_BLOCK) void k_face(phd_cells c, phd_faces f,
-                                                    const double* __restrict__ frec,
-                                                    const double* __restrict__ rec, int ld,
-                                                    Gas gas, double dt_half, double dt_full,
-                                                    int boost, int gizmo, FaceArrays flux,
-                                                    double* __restrict__ g,
-                                                    unsigned long long* err)
+// BATCH: the time step is per problem, dt_full = bdt[fprob[m]] and dt_half =
+// 0.5 * dt_full (the arguments are ignored), and the mesh is static (face
+// velocities zero, not read).
+template <int D, int ORDER, int SOLVER, int OUT, int BATCH = 0, int BS = PHD_BLOCK, int XCD = 1>
+__global__ __launch_bounds__(BS) void k_face(phd_cells c, phd_faces f,
+                                             const double* __restrict__ frec,
+                                             const double* __restrict__ rec, int ld,
+                                             Gas gas, double dt_half, double dt_full,
+                                             int boost, int gizmo, FaceArrays flux,
+                                             double* __restrict__ g,
+                                             unsigned long long* err,
+                                             const double* __restrict__ bdt,
+                                             const int32_t* __restrict__ fprob)
 {
     constexpr int NF = D + 2;
-    long long m = xcd_index(f.n_faces);
+    long long m = sched_index<BS, XCD>(f.n_faces);
     if (m >= f.n_faces) return;
     const int i = f.pair_i[m], j = f.pair_j[m];
+    if (BATCH) {
+        dt_full = bdt[fprob[m]];
+        dt_half = 0.5 * dt_full;
+        if (dt_full == 0.0) {              // a frozen problem: no change
+            constexpr int GW = summand_width<D>();
+            double2* o = reinterpret_cast<double2*>(g + (size_t)m * GW);
+#pragma unroll
+            for (int q = 0; q < GW / 2; q++) o[q] = make_double2(0.0, 0.0);
+            return;
+        }
+    }
     double com[D], n[D], area, diff, fv[D], L[NF], R[NF], F[NF];
     load_face_rec<D>(frec, m, com, area, n, diff);
 #pragma unroll
-    for (int k = 0; k < D; k++) fv[k] = f.velocity[k][m];
+    for (int k = 0; k < D; k++) fv[k] = BATCH ? 0.0 : f.velocity[k][m];   // batches: static meshes
     face_states<D, ORDER>(c, rec, ld, i, j, com, diff, fv, gas.gamma, dt_half, boost, gizmo,
                           PHD_STAGE_SPATIAL | PHD_STAGE_TEMPORAL, L, R);
     if (!solve_face<D, SOLVER>(L, R, n, fv, gas, boost, F)) {
@@ -825,8 +849,8 @@ __device__ __forceinline__ void load_summand(const double* g, int fid, double* s
     }
 }
 
-template <int D>
-__global__ __launch_bounds__(PHD_BLOCK) void k_update_sell(phd_cells c,
+template <int D, int BS = PHD_BLOCK, int XCD = 1>
+__global__ __launch_bounds__(BS) void k_update_sell(phd_cells c,
                                                            const double* __restrict__ g,
                                                            int owner_runs,
                                                            const int2* __restrict__ csr,
@@ -838,7 +862,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_update_sell(phd_cells c,
                                                            unsigned long long* err)
 {
     constexpr int NF = D + 2, GW = summand_width<D>(), B = 4;
-    const long long t = xcd_index(c.n_cells);
+    const long long t = sched_index<BS, XCD>(c.n_cells);
     if (t >= c.n_cells) return;
     const int i = perm[t];
     const int beg = off[i], end = off[i + 1];
@@ -900,59 +924,122 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_update_sell(phd_cells c,
 // over all cells; optionally refreshes the packed [position | primitive]
 // rows.  Ghosts read only their image's conserved values, which this kernel
 // does not write.
+//
+// BATCH (a union of problems, problem p owning cells [cs[p], cs[p+1])): also
+// folds each real cell's CFL bound into dtbits[p] (atomic min on the bit
+// pattern: exact) and flags bad[p] when the cell's mass, energy, density or
+// pressure is not positive; a wave reduces per problem first, one atomic
+// per problem it touches.
 template <int D>
+__device__ __forceinline__ void fold_cfl(bool real, int i, const double* u, const double* p,
+                                         double vol, double gamma, int boost,
+                                         const int32_t* __restrict__ cs, int n_problems,
+                                         unsigned long long* dtbits, int32_t* bad)
+{
+    double v = __longlong_as_double((long long)INF_BITS);
+    int b = 0, pid = -1;
+    if (real) {
+        const double cs_ = sqrt(gamma * p[D + 1] / p[0]);
+        const double R = cell_radius<D>(vol);
+        if (boost) {
+            v = R / cs_;
+        } else {
+            double vsq = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; k++) vsq += p[1 + k] * p[1 + k];
+            v = R / (cs_ + sqrt(vsq));
+        }
+        if (!(v >= 0.0)) v = __longlong_as_double((long long)INF_BITS);
+        b = !(u[0] > 0.0) || !(u[D + 1] > 0.0) || !(p[0] > 0.0) || !(p[D + 1] > 0.0);
+        int lo = 0, hi = n_problems - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (cs[mid] <= i) lo = mid; else hi = mid - 1;
+        }
+        pid = lo;
+    }
+    // distinct problems of the wave, smallest first
+    int todo = pid;
+    for (;;) {
+        int q = todo < 0 ? 0x7fffffff : todo;
+        for (int off = 32; off > 0; off >>= 1) q = min(q, __shfl_xor(q, off, 64));
+        if (q == 0x7fffffff) break;
+        double m = pid == q ? v : __longlong_as_double((long long)INF_BITS);
+        int bb = pid == q ? b : 0;
+        for (int off = 32; off > 0; off >>= 1) {
+            m = fmin(m, __shfl_xor(m, off, 64));
+            bb |= __shfl_xor(bb, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            if (m < __longlong_as_double((long long)INF_BITS))
+                atomicMin(&dtbits[q], (unsigned long long)__double_as_longlong(m));
+            if (bb) atomicOr(&bad[q], 1);
+        }
+        if (pid == q) todo = -1;
+    }
+}
+
+template <int D, int BATCH = 0>
 __global__ __launch_bounds__(PHD_BLOCK) void k_ghost_refresh_eos(phd_cells c, int reflect,
                                                                  double lo0, double lo1,
                                                                  double lo2, double hi0,
                                                                  double hi1, double hi2,
-                                                                 double gamma, double* xp)
+                                                                 double gamma, double* xp,
+                                                                 const int32_t* __restrict__ bcs,
+                                                                 int n_problems, int boost,
+                                                                 unsigned long long* dtbits,
+                                                                 int32_t* bad)
 {
     constexpr int NF = D + 2;
-    long long ii = xcd_index(c.n_cells);
-    if (ii >= c.n_cells) return;
-    const int i = (int)ii;
-    double u[NF];
-    if (c.tag[i] == 1) {
-        const int s = c.map[i];
+    const long long ii = xcd_index(c.n_cells);
+    const bool in = ii < c.n_cells;      // no early exit: the BATCH fold is wave-wide
+    const int i = in ? (int)ii : 0;
+    double u[NF], p[NF], vol = 0.0;
+    bool real = false;
+    if (in) {
+        if (c.tag[i] == 1) {
+            const int s = c.map[i];
 #pragma unroll
-        for (int q = 0; q < NF; q++) u[q] = c.cons[q][s];
-        if (reflect && c.type[i] == 2) {
-            const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+            for (int q = 0; q < NF; q++) u[q] = c.cons[q][s];
+            if (reflect && c.type[i] == 2) {
+                const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
 #pragma unroll
-            for (int k = 0; k < D; k++) {
-                double xk = c.x[k][i];
-                if (xk < lo[k]) u[1 + k] *= -1;
-                if (xk > hi[k]) u[1 + k] *= -1;
+                for (int k = 0; k < D; k++) {
+                    double xk = c.x[k][i];
+                    if (xk < lo[k]) u[1 + k] *= -1;
+                    if (xk > hi[k]) u[1 + k] *= -1;
+                }
             }
+#pragma unroll
+            for (int q = 0; q < NF; q++) c.cons[q][i] = u[q];
+        } else {
+            real = true;
+#pragma unroll
+            for (int q = 0; q < NF; q++) u[q] = c.cons[q][i];
         }
+        vol = c.volume[i];
+        p[0] = u[0] / vol;
+        double v_sq = 0.;
 #pragma unroll
-        for (int q = 0; q < NF; q++) c.cons[q][i] = u[q];
-    } else {
+        for (int k = 0; k < D; k++) {
+            p[1 + k] = u[1 + k] / u[0];
+            v_sq += p[1 + k] * p[1 + k];
+        }
+        p[D + 1] = (u[D + 1] / vol - .5 * p[0] * v_sq) * (gamma - 1.);
 #pragma unroll
-        for (int q = 0; q < NF; q++) u[q] = c.cons[q][i];
+        for (int q = 0; q < NF; q++) c.prim[q][i] = p[q];
+        if (xp) {
+            double r[XPW] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < D; k++) r[k] = c.x[k][i];
+#pragma unroll
+            for (int n = 0; n < NF; n++) r[D + n] = p[n];
+            double2* o = reinterpret_cast<double2*>(xp + (size_t)i * XPW);
+#pragma unroll
+            for (int q = 0; q < XPW / 2; q++) o[q] = make_double2(r[2 * q], r[2 * q + 1]);
+        }
     }
-    const double vol = c.volume[i];
-    double p[NF];
-    p[0] = u[0] / vol;
-    double v_sq = 0.;
-#pragma unroll
-    for (int k = 0; k < D; k++) {
-        p[1 + k] = u[1 + k] / u[0];
-        v_sq += p[1 + k] * p[1 + k];
-    }
-    p[D + 1] = (u[D + 1] / vol - .5 * p[0] * v_sq) * (gamma - 1.);
-#pragma unroll
-    for (int q = 0; q < NF; q++) c.prim[q][i] = p[q];
-    if (xp) {
-        double r[XPW] = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-        for (int k = 0; k < D; k++) r[k] = c.x[k][i];
-#pragma unroll
-        for (int n = 0; n < NF; n++) r[D + n] = p[n];
-        double2* o = reinterpret_cast<double2*>(xp + (size_t)i * XPW);
-#pragma unroll
-        for (int q = 0; q < XPW / 2; q++) o[q] = make_double2(r[2 * q], r[2 * q + 1]);
-    }
+    if (BATCH) fold_cfl<D>(real, i, u, p, vol, gamma, boost, bcs, n_problems, dtbits, bad);
 }
 
 // ---------------------------------------------------------------------------
@@ -1058,7 +1145,7 @@ int launch_face_o(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_face
 {
     const unsigned nb = xcd_grid(f->n_faces);
     switch (solver) {
-#define FACE(S) k_face<D, ORDER, S, OUT><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, p->face_rec, rec, ld, make_gas(gamma), dt_half, dt_full, boost, gizmo, flux, p->g, h->d_err)
+#define FACE(S) k_face<D, ORDER, S, OUT><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, p->face_rec, rec, ld, make_gas(gamma), dt_half, dt_full, boost, gizmo, flux, p->g, h->d_err, nullptr, nullptr)
     case PHD_SOLVER_HLL: FACE(PHD_SOLVER_HLL); break;
     case PHD_SOLVER_HLLC: FACE(PHD_SOLVER_HLLC); break;
     case PHD_SOLVER_EXACT: FACE(PHD_SOLVER_EXACT); break;
@@ -1565,10 +1652,423 @@ int phd_ghost_refresh_eos(phd_handle* h, phd_plan* p, const phd_cells* c, int bo
         return phd_pack_cells(h, p, c);
     }
     if (c->dim == 2)
-        k_ghost_refresh_eos<2><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, refl, l[0], l[1], l[2], u[0], u[1], u[2], gamma, xp);
+        k_ghost_refresh_eos<2><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, refl, l[0], l[1], l[2], u[0], u[1], u[2], gamma, xp, nullptr, 0, 0, nullptr, nullptr);
     else
-        k_ghost_refresh_eos<3><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, refl, l[0], l[1], l[2], u[0], u[1], u[2], gamma, xp);
+        k_ghost_refresh_eos<3><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(*c, refl, l[0], l[1], l[2], u[0], u[1], u[2], gamma, xp, nullptr, 0, 0, nullptr, nullptr);
     LAUNCHCHK();
+    return 0;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched ensembles: many independent problems in one union mesh (problem p
+// owns cells [cell_start[p], cell_start[p+1]) and a run of faces).  Every
+// kernel is flat over the union; the time step is per problem: one block per
+// problem takes the CFL minimum over its real cells, and the face kernel
+// reads its problem's dt through a per-face problem id.
+
+struct phd_batch {
+    int dim, n_problems, n_cells, n_faces;
+    int32_t* cell_start;    // n_problems + 1
+    int32_t* fprob;         // per face
+    double* dt;             // per problem
+    double* hist;           // hist_cap x n_problems
+    long long hist_cap;
+    int32_t* counter;       // steps recorded since the last reset
+    int32_t* fault;         // per problem: 0, or 1 + the step count when it was frozen
+    unsigned long long* dtbits;  // per problem: CFL minimum folded by the refresh (bits)
+    int32_t* badp;          // per problem: a non-positive state seen by the refresh
+    int32_t* bad;           // creation check: faces joining two problems
+    hipStream_t stream;
+    hipEvent_t ev_in, ev_out;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    phd_schedule sched;
+    void* base;
+    size_t bytes;
+};
+
+namespace {
+
+// One thread per problem, at the start of a step: dt[p] = cfl * the CFL
+// minimum the last refresh folded (riemann.pyx:120-188), or 0 for a frozen
+// problem: one whose state the refresh found with a non-positive mass,
+// energy, density or pressure (the previous update failed, where the
+// reference raises for its single problem) is frozen from then on and
+// recorded in fault[p]; the other problems go on.  Clears the accumulators.
+__global__ __launch_bounds__(PHD_BLOCK) void k_batch_dt(int n_problems, double cfl,
+                                                        unsigned long long* __restrict__ dtbits,
+                                                        int32_t* __restrict__ badp,
+                                                        int32_t* __restrict__ fault,
+                                                        double* __restrict__ dt,
+                                                        double* __restrict__ hist,
+                                                        long long hist_cap,
+                                                        const int32_t* __restrict__ counter)
+{
+    const int p = blockIdx.x * PHD_BLOCK + threadIdx.x;
+    if (p >= n_problems) return;
+    const int k = counter[0];
+    if (badp[p] && fault[p] == 0) fault[p] = k + 1;
+    const double d = fault[p] ? 0.0 : cfl * __longlong_as_double((long long)dtbits[p]);
+    dt[p] = d;
+    if (hist && k < hist_cap) hist[(long long)k * n_problems + p] = d;
+    dtbits[p] = INF_BITS;
+    badp[p] = 0;
+}
+
+__global__ __launch_bounds__(PHD_BLOCK) void k_batch_acc_clear(int n_problems,
+                                                               unsigned long long* dtbits,
+                                                               int32_t* badp)
+{
+    const int p = blockIdx.x * PHD_BLOCK + threadIdx.x;
+    if (p >= n_problems) return;
+    dtbits[p] = INF_BITS;
+    badp[p] = 0;
+}
+
+__global__ void k_batch_tick(int32_t* counter)
+{
+    if (threadIdx.x == 0) counter[0] += 1;
+}
+
+__global__ void k_batch_zero(int32_t* counter)
+{
+    if (threadIdx.x == 0) counter[0] = 0;
+}
+
+__global__ void k_batch_clear(int32_t* a, int n)
+{
+    const int i = blockIdx.x * PHD_BLOCK + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+// problem of each face (binary search of pair_i in cell_start); flags a
+// face whose pair_j lies in another problem
+__global__ __launch_bounds__(PHD_BLOCK) void k_batch_fprob(phd_faces f, int n_problems,
+                                                           const int32_t* __restrict__ cell_start,
+                                                           int32_t* __restrict__ fprob,
+                                                           int32_t* bad)
+{
+    long long m = xcd_index(f.n_faces);
+    if (m >= f.n_faces) return;
+    const int i = f.pair_i[m], j = f.pair_j[m];
+    int lo = 0, hi = n_problems - 1;
+    while (lo < hi) {                    // largest p with cell_start[p] <= i
+        const int mid = (lo + hi + 1) >> 1;
+        if (cell_start[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    fprob[m] = lo;
+    if (j < cell_start[lo] || j >= cell_start[lo + 1]) bad[0] = 1;
+}
+
+// the step's three main kernels under a schedule (block size, XCD remap)
+#define PHD_SCHED_CASES(X)                                    \
+    X(64, 0) X(64, 1) X(128, 0) X(128, 1) X(256, 0) X(256, 1)
+
+template <int D>
+int batch_grad(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_schedule& s, double* rec,
+               int ld)
+{
+    if (p->bad_cell >= 0) {
+        k_set_err<<<1, 64, 0, h->stream>>>(h->d_err, 1, (unsigned long long)p->bad_cell);
+        LAUNCHCHK();
+    }
+    const long long n = c->n_cells;
+#define X(BS, XC)                                                                              \
+    if (s.grad_block == BS && s.xcd == XC) {                                                   \
+        k_grad_sell<D, PHD_LIMITER_AREPO, BS, XC><<<sched_grid<BS, XC>(n), BS, 0, h->stream>>>( \
+            c->n_cells, c->tag, p->cell_geo, p->xp, p->off, p->perm, p->sbase, p->sj,          \
+            p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], rec, ld);                          \
+        LAUNCHCHK();                                                                           \
+        return 0;                                                                              \
+    }
+    PHD_SCHED_CASES(X)
+#undef X
+    return PHD_E_ARG;
+}
+
+template <int D>
+int batch_face(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c, const phd_faces* f,
+               const double* rec, int ld, double gamma, int boost, int gizmo)
+{
+    const phd_schedule& s = b->sched;
+    const long long n = f->n_faces;
+    if (n == 0) return 0;
+    const FaceArrays none = fa(nullptr, 0);
+#define X(BS, XC)                                                                              \
+    if (s.face_block == BS && s.xcd == XC) {                                                   \
+        k_face<D, PHD_ORDER_LINEAR, PHD_SOLVER_HLLC, 2, 1, BS, XC>                              \
+            <<<sched_grid<BS, XC>(n), BS, 0, h->stream>>>(*c, *f, p->face_rec, rec, ld,         \
+                                                          make_gas(gamma), 0.0, 0.0, boost,    \
+                                                          gizmo, none, p->g, h->d_err, b->dt,  \
+                                                          b->fprob);                           \
+        LAUNCHCHK();                                                                           \
+        return 0;                                                                              \
+    }
+    PHD_SCHED_CASES(X)
+#undef X
+    return PHD_E_ARG;
+}
+
+template <int D>
+int batch_update(phd_handle* h, const phd_schedule& s, phd_plan* p, const phd_cells* c)
+{
+    const long long n = c->n_cells;
+#define X(BS, XC)                                                                              \
+    if (s.update_block == BS && s.xcd == XC) {                                                 \
+        k_update_sell<D, BS, XC><<<sched_grid<BS, XC>(n), BS, 0, h->stream>>>(                 \
+            *c, p->g, p->owner_runs, p->csr, p->off, p->perm, p->sbase, p->sfi, p->cinfo,      \
+            h->d_err);                                                                         \
+        LAUNCHCHK();                                                                           \
+        return 0;                                                                              \
+    }
+    PHD_SCHED_CASES(X)
+#undef X
+    return PHD_E_ARG;
+}
+#undef PHD_SCHED_CASES
+
+bool sched_ok(const phd_schedule& s)
+{
+    auto ok = [](int v) { return v == 64 || v == 128 || v == 256; };
+    return ok(s.grad_block) && ok(s.face_block) && ok(s.update_block) && (s.xcd == 0 || s.xcd == 1);
+}
+
+void drop_graph(phd_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// ghost refresh + EOS of the union, folding the per-problem CFL minimum and
+// fault flags for the next step
+template <int D>
+int batch_refresh(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c, int boundary,
+                  const double* lo, const double* hi, double gamma, int boost)
+{
+    double l[3], u[3];
+    bounds3(D, boundary, lo, hi, l, u);
+    k_ghost_refresh_eos<D, 1><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(
+        *c, boundary == PHD_BOUNDARY_REFLECTIVE, l[0], l[1], l[2], u[0], u[1], u[2], gamma,
+        p->xp, b->cell_start, b->n_problems, boost, b->dtbits, b->badp);
+    LAUNCHCHK();
+    return 0;
+}
+
+unsigned pblocks(int n) { return (unsigned)((n + PHD_BLOCK - 1) / PHD_BLOCK); }
+
+template <int D>
+int batch_step(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c, const phd_faces* f,
+               double* rec, int ld, int boundary, const double* lo, const double* hi,
+               double gamma, double cfl, int boost, int gizmo, int flags)
+{
+    k_batch_dt<<<pblocks(b->n_problems), PHD_BLOCK, 0, h->stream>>>(
+        b->n_problems, cfl, b->dtbits, b->badp, b->fault, b->dt, b->hist, b->hist_cap, b->counter);
+    LAUNCHCHK();
+    int rc = 0;
+    if (!(flags & PHD_CORE_XP_FRESH)) rc = phd_pack_cells(h, p, c);
+    if (!rc) rc = batch_grad<D>(h, p, c, b->sched, rec, ld);
+    if (!rc) rc = phd_update_ghost_gradients(h, p, c, boundary, lo, hi, rec, ld);
+    if (!rc) rc = batch_face<D>(h, b, p, c, f, rec, ld, gamma, boost, gizmo);
+    if (!rc) rc = batch_update<D>(h, b->sched, p, c);
+    if (!rc) rc = batch_refresh<D>(h, b, p, c, boundary, lo, hi, gamma, boost);
+    if (rc) return rc;
+    k_batch_tick<<<1, 64, 0, h->stream>>>(b->counter);
+    LAUNCHCHK();
+    return 0;
+}
+
+int batch_args_ok(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c, const phd_faces* f,
+                  const double* rec, int ld, int boundary, const double* lo, const double* hi)
+{
+    if (!h || !b || check_cells(c) || check_faces(f) || check_plan(p, c) ||
+        check_plan_faces(p, f) || check_rec(c->dim, rec, ld))
+        return PHD_E_ARG;
+    if (b->dim != c->dim || b->n_cells != c->n_cells || b->n_faces != f->n_faces) return PHD_E_ARG;
+    if (boundary != PHD_BOUNDARY_REFLECTIVE && boundary != PHD_BOUNDARY_PERIODIC) return PHD_E_MODE;
+    if (boundary == PHD_BOUNDARY_REFLECTIVE && (!lo || !hi)) return PHD_E_ARG;
+    if (!c->map) return PHD_E_ARG;       // ghosts are refreshed from their images
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int phd_batch_create(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces* f,
+                     int32_t n_problems, const int32_t* cell_start, int64_t hist_steps,
+                     phd_batch** out)
+{
+    if (!h || !out || check_cells(c) || check_faces(f) || check_plan(p, c) ||
+        check_plan_faces(p, f) || n_problems < 1 || !cell_start || hist_steps < 0)
+        return PHD_E_ARG;
+    *out = nullptr;
+    if (cell_start[0] != 0 || cell_start[n_problems] != c->n_cells) return PHD_E_ARG;
+    for (int q = 0; q < n_problems; q++)
+        if (cell_start[q + 1] <= cell_start[q]) return PHD_E_ARG;
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t s_cs = al(4 * (size_t)(n_problems + 1)), s_fp = al(4 * (size_t)f->n_faces + 4);
+    const size_t s_dt = al(8 * (size_t)n_problems), s_h = al(8 * (size_t)hist_steps * (size_t)n_problems);
+    const size_t s_ct = al(8), s_fa = al(4 * (size_t)n_problems), s_db = al(8 * (size_t)n_problems);
+    phd_batch* b = new phd_batch();
+    b->dim = c->dim;
+    b->n_problems = n_problems;
+    b->n_cells = c->n_cells;
+    b->n_faces = f->n_faces;
+    b->hist_cap = hist_steps;
+    b->sched = phd_schedule{256, 256, 256, 1};
+    b->bytes = s_cs + s_fp + s_dt + s_h + 2 * s_ct + 2 * s_fa + s_db;
+    if (hipMalloc(&b->base, b->bytes) != hipSuccess) { delete b; return PHD_E_HIP; }
+    char* q = (char*)b->base;
+    b->cell_start = (int32_t*)q; q += s_cs;
+    b->fprob = (int32_t*)q; q += s_fp;
+    b->dt = (double*)q; q += s_dt;
+    b->hist = hist_steps > 0 ? (double*)q : nullptr; q += s_h;
+    b->counter = (int32_t*)q; q += s_ct;
+    b->fault = (int32_t*)q; q += s_fa;
+    b->dtbits = (unsigned long long*)q; q += s_db;
+    b->badp = (int32_t*)q; q += s_fa;
+    b->bad = (int32_t*)q;
+    int32_t hbad = 0;
+    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
+    if (ok) ok = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) == hipSuccess;
+    if (ok) ok = hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipMemcpyAsync(b->cell_start, cell_start, 4 * (size_t)(n_problems + 1),
+                              hipMemcpyHostToDevice, h->stream) == hipSuccess;
+    if (ok) {
+        k_batch_zero<<<1, 64, 0, h->stream>>>(b->counter);
+        k_batch_zero<<<1, 64, 0, h->stream>>>(b->bad);
+        k_batch_clear<<<(n_problems + PHD_BLOCK - 1) / PHD_BLOCK, PHD_BLOCK, 0, h->stream>>>(b->fault, n_problems);
+        k_batch_acc_clear<<<pblocks(n_problems), PHD_BLOCK, 0, h->stream>>>(n_problems, b->dtbits, b->badp);
+        if (f->n_faces > 0)
+            k_batch_fprob<<<xcd_grid(f->n_faces), PHD_BLOCK, 0, h->stream>>>(*f, n_problems, b->cell_start,
+                                                                           b->fprob, b->bad);
+        ok = hipPeekAtLastError() == hipSuccess;
+    }
+    ok = ok && hipMemcpyAsync(&hbad, b->bad, 4, hipMemcpyDeviceToHost, h->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
+    if (!ok || hbad) {
+        (void)hipGetLastError();
+        phd_batch_destroy(b);
+        return ok ? PHD_E_ARG : PHD_E_HIP;
+    }
+    *out = b;
+    return 0;
+}
+
+int phd_batch_destroy(phd_batch* b)
+{
+    if (!b) return PHD_E_ARG;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+    if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    (void)hipFree(b->base);
+    delete b;
+    return 0;
+}
+
+int phd_batch_set_schedule(phd_batch* b, const phd_schedule* s)
+{
+    if (!b || !s || !sched_ok(*s)) return PHD_E_ARG;
+    drop_graph(b);
+    b->sched = *s;
+    return 0;
+}
+
+int phd_batch_step(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c,
+                   const phd_faces* f, double* rec, int32_t rec_ld, int boundary,
+                   const double* lo, const double* hi, double gamma, double cfl, int boost,
+                   int gizmo, int flags)
+{
+    const int rc = batch_args_ok(h, b, p, c, f, rec, rec_ld, boundary, lo, hi);
+    if (rc) return rc;
+    if (flags & ~PHD_CORE_XP_FRESH) return PHD_E_ARG;
+    return c->dim == 2 ? batch_step<2>(h, b, p, c, f, rec, rec_ld, boundary, lo, hi, gamma, cfl, boost, gizmo, flags)
+                       : batch_step<3>(h, b, p, c, f, rec, rec_ld, boundary, lo, hi, gamma, cfl, boost, gizmo, flags);
+}
+
+int phd_batch_capture(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c,
+                      const phd_faces* f, double* rec, int32_t rec_ld, int boundary,
+                      const double* lo, const double* hi, double gamma, double cfl, int boost,
+                      int gizmo)
+{
+    int rc = batch_args_ok(h, b, p, c, f, rec, rec_ld, boundary, lo, hi);
+    if (rc) return rc;
+    drop_graph(b);
+    phd_handle hc = *h;            // same error words, the batch's stream
+    hc.stream = b->stream;
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    rc = c->dim == 2 ? batch_step<2>(&hc, b, p, c, f, rec, rec_ld, boundary, lo, hi, gamma, cfl, boost, gizmo, PHD_CORE_XP_FRESH)
+                     : batch_step<3>(&hc, b, p, c, f, rec, rec_ld, boundary, lo, hi, gamma, cfl, boost, gizmo, PHD_CORE_XP_FRESH);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return rc ? rc : PHD_E_HIP;
+    }
+    b->graph = g;
+    if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        drop_graph(b);
+        return PHD_E_HIP;
+    }
+    return 0;
+}
+
+int phd_batch_launch(phd_handle* h, phd_batch* b, int32_t n_steps)
+{
+    if (!h || !b || !b->exec || n_steps < 0) return PHD_E_ARG;
+    HIPCHK(hipEventRecord(b->ev_in, h->stream));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->ev_in, 0));
+    for (int s = 0; s < n_steps; s++) HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    HIPCHK(hipEventRecord(b->ev_out, b->stream));
+    HIPCHK(hipStreamWaitEvent(h->stream, b->ev_out, 0));
+    return 0;
+}
+
+int phd_batch_refresh(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c,
+                      int boundary, const double* lo, const double* hi, double gamma, int boost)
+{
+    if (!h || !b || check_cells(c) || check_plan(p, c)) return PHD_E_ARG;
+    if (b->dim != c->dim || b->n_cells != c->n_cells || !c->map) return PHD_E_ARG;
+    if (boundary != PHD_BOUNDARY_REFLECTIVE && boundary != PHD_BOUNDARY_PERIODIC) return PHD_E_MODE;
+    if (boundary == PHD_BOUNDARY_REFLECTIVE && (!lo || !hi)) return PHD_E_ARG;
+    k_batch_acc_clear<<<pblocks(b->n_problems), PHD_BLOCK, 0, h->stream>>>(b->n_problems, b->dtbits, b->badp);
+    LAUNCHCHK();
+    return c->dim == 2 ? batch_refresh<2>(h, b, p, c, boundary, lo, hi, gamma, boost)
+                       : batch_refresh<3>(h, b, p, c, boundary, lo, hi, gamma, boost);
+}
+
+int phd_batch_reset_history(phd_handle* h, phd_batch* b)
+{
+    if (!h || !b) return PHD_E_ARG;
+    k_batch_zero<<<1, 64, 0, h->stream>>>(b->counter);
+    k_batch_clear<<<(b->n_problems + PHD_BLOCK - 1) / PHD_BLOCK, PHD_BLOCK, 0, h->stream>>>(b->fault, b->n_problems);
+    LAUNCHCHK();
+    return 0;
+}
+
+int phd_batch_read(phd_handle* h, phd_batch* b, double* dt, int64_t* steps, double* history,
+                   int32_t* faults)
+{
+    if (!h || !b) return PHD_E_ARG;
+    int32_t k = 0;
+    HIPCHK(hipMemcpyAsync(&k, b->counter, 4, hipMemcpyDeviceToHost, h->stream));
+    if (dt) HIPCHK(hipMemcpyAsync(dt, b->dt, 8 * (size_t)b->n_problems, hipMemcpyDeviceToHost, h->stream));
+    if (faults) HIPCHK(hipMemcpyAsync(faults, b->fault, 4 * (size_t)b->n_problems, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (steps) *steps = k;
+    if (history && b->hist && k > 0) {
+        const long long n = (k < b->hist_cap ? k : b->hist_cap) * (long long)b->n_problems;
+        HIPCHK(hipMemcpyAsync(history, b->hist, 8 * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
     return 0;
 }
 
