@@ -194,3 +194,42 @@ def core_step(a, dt, solver=1, boost=True, gizmo=True, limiter=0):
                              _pp(flux), ctypes.byref(bad))
     return {"grad": grad, "left": pl, "right": pr, "flux": flux, "cons": cons, "rc": rc,
             "bad": bad.value}
+
+
+DUAL_PATH = os.path.join(HERE, "libphd_oracle_dual.so")
+_dual = None
+
+
+def dual_lib():
+    global _dual
+    if _dual is None:
+        if not os.path.exists(DUAL_PATH):
+            build()
+        _dual = ctypes.CDLL(DUAL_PATH)
+    return _dual
+
+
+def rollout_dual(a, prim0, dprim0, dts, boost=True, gizmo=True):
+    """Static-mesh rollout (oracle_dual.cpp: ord_rollout) with tangents:
+    initial primitives prim0 (all cells) and their tangents dprim0; time
+    steps dts.  Returns (final primitives, their tangents), or raises on a
+    fault."""
+    dim, n, nf = a["dim"], a["n"], a["nf"]
+    prim0, dprim0 = _f64(prim0), _f64(dprim0)
+    out = [np.zeros(n) for _ in range(dim + 2)]
+    dout = [np.zeros(n) for _ in range(dim + 2)]
+    dt = np.ascontiguousarray(dts, dtype=np.float64)
+    lo = np.array(a["lo"] + [0.0] * (3 - dim))
+    hi = np.array(a["hi"] + [0.0] * (3 - dim))
+    x, dcom = _f64(a["x"]), _f64(a["dcom"])
+    com, normal = _f64(a["com"]), _f64(a["normal"])
+    rc = dual_lib().ord_rollout(dim, n, nf, ctypes.c_double(a["gamma"]), len(dt), _p(dt),
+                                int(boost), int(gizmo), int(a["boundary"]), _p(a["tag"]),
+                                _p(a["type"]), _p(a["map"]), _pp(x), _pp(dcom),
+                                _p(a["volume"]), _p(a["area"]), _pp(com), _pp(normal),
+                                _p(a["pair_i"]), _p(a["pair_j"]), _p(a["nbr_off"]),
+                                _p(a["nbr_face"]), _p(lo), _p(hi), _pp(prim0), _pp(dprim0),
+                                _pp(out), _pp(dout))
+    if rc != 0:
+        raise RuntimeError("oracle rollout fault %d" % rc)
+    return out, dout

@@ -217,3 +217,55 @@ def test_rollout_matches_plain_reverse_sweep(gpu):
         print("every %d: max |d rho| %.3e, max |d vy| %.3e" %
               (every, float((gr - ref[0]).abs().max()), float((gv - ref[1]).abs().max())))
         assert torch.equal(gr, ref[0]) and torch.equal(gv, ref[1]), every
+
+
+@pytest.mark.parametrize("name,n,pressure,n_steps,every", [("kh_2d", 16, None, 6, 2),
+                                                          ("sod_2d", 16, None, 5, 2),
+                                                          ("sedov_3d", 6, 1.0, 4, 2)])
+def test_rollout_against_forward_mode_ad_of_the_oracle(gpu, name, n, pressure, n_steps, every):
+    """The taped reference: the oracle differentiated in forward mode
+    (oracle/oracle_dual.cpp) gives the exact directional derivative of the
+    same rollout for the branch pattern of the state, with no truncation
+    error and independent of the hand-written reverse mode.  The GPU
+    gradient contracted with a random direction must equal it, and the
+    oracle's final state must equal the GPU's."""
+    from oracle import oracle
+    from phd import problems
+    from phd.adjoint import forward_dts, rollout
+    st = _stepper(name, n, pressure=pressure)
+    real = st.real
+    pc = st.p.particles
+    rho0 = st.prim_init[0].clone()
+    v0 = st.prim_init[2].clone()
+    _, dts = forward_dts(st, rho0, v0, n_steps)
+    rng = np.random.default_rng(17)
+    nr = int(real.sum())
+    w = torch.as_tensor(rng.normal(size=nr), device=gpu)
+    r0 = rho0.clone().requires_grad_(True)
+    vv = v0.clone().requires_grad_(True)
+    rho = rollout(st, r0, vv, n_steps, every=every, dts=dts)
+    (rho * w).sum().backward()
+    g_rho, g_v = r0.grad.cpu().numpy(), vv.grad.cpu().numpy()
+    rho_gpu = rho.detach().cpu().numpy()
+
+    a = problems.host_arrays(st.p)
+    na = a["n"]
+    realn = real.cpu().numpy()
+    prim0 = [np.zeros(na) for _ in range(st.nf)]
+    for q in range(st.nf):
+        prim0[q][realn] = st.prim_init[q].cpu().numpy()
+    wn = w.cpu().numpy()
+    for trial in range(3):
+        d_rho = rng.normal(size=nr)
+        d_v = rng.normal(size=nr) * 0.1
+        dprim = [np.zeros(na) for _ in range(st.nf)]
+        dprim[0][realn] = d_rho
+        dprim[2][realn] = d_v
+        out, dout = oracle.rollout_dual(a, prim0, dprim, dts, boost=bool(st.boost),
+                                        gizmo=bool(st.gizmo))
+        if trial == 0:
+            assert np.allclose(out[0][realn], rho_gpu, rtol=1e-12, atol=0.0)
+        ad = float(np.dot(wn, dout[0][realn]))
+        adj = float(np.dot(g_rho, d_rho) + np.dot(g_v, d_v))
+        print("%s trial %d forward-mode %.14e adjoint %.14e" % (name, trial, ad, adj))
+        assert abs(ad - adj) <= 1e-10 * max(abs(ad), abs(adj)), (ad, adj)
