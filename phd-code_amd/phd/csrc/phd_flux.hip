@@ -515,57 +515,74 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
     }
     const int deg = off[i + 1] - off[i];
     const long long base = sbase[t / SLICE] + (t % SLICE);
-    for (int e = 0; e < deg; e++) {
-        const SlotEnt<D> cur = load_slot<D>(sj, sx, sy, sz, sa, base + (long long)e * SLICE);
-        const int j = cur.j;
-        if (j == INVALID) continue;
-        const double* com = cur.com;
-        const double area = cur.area;
-        double r[XPW];
-        const long long lj = (long long)j - w0;
-        if (STAGE && lj >= 0 && lj < SORT_WINDOW) {
+    // GB entries in flight: their slot loads, then their row reads, then the
+    // entries applied one by one in list order (the same sums).  Two pay in
+    // 2D; in 3D the extra registers cost a wave per SIMD
+    constexpr int GB = D == 2 ? 2 : 1, LB = 4;
+    for (int e0 = 0; e0 < deg; e0 += GB) {
+        SlotEnt<D> cur[GB];
+        double rb[GB][XPW];
 #pragma unroll
-            for (int q = 0; q < XPW / 2; q++) {
-                const double2 v = rows[lj * (XPW / 2) + q];
-                r[2 * q] = v.x;
-                r[2 * q + 1] = v.y;
+        for (int b = 0; b < GB; b++) {
+            cur[b].j = INVALID;
+            if (e0 + b < deg) cur[b] = load_slot<D>(sj, sx, sy, sz, sa, base + (long long)(e0 + b) * SLICE);
+        }
+#pragma unroll
+        for (int b = 0; b < GB; b++) {
+            const int j = cur[b].j;
+            if (j == INVALID) continue;
+            const long long lj = (long long)j - w0;
+            if (STAGE && lj >= 0 && lj < SORT_WINDOW) {
+#pragma unroll
+                for (int q = 0; q < XPW / 2; q++) {
+                    const double2 v = rows[lj * (XPW / 2) + q];
+                    rb[b][2 * q] = v.x;
+                    rb[b][2 * q + 1] = v.y;
+                }
+            } else {
+                load_row(xp, j, rb[b]);
             }
-        } else {
-            load_row(xp, j, r);
         }
-        double cfx[D], dr[D], rr = 0.0;
 #pragma unroll
-        for (int k = 0; k < D; k++) {
-            const double xj = r[k];
-            cfx[k] = com[k] - 0.5 * (xi[k] + xj);
-            dr[k] = xi[k] - xj;
-            rr += dr[k] * dr[k];
-        }
-        rr = sqrt(rr);
-        const double rv = rr * vol;
-        // Springel (2010) eq. 21 summand, area*(...)/(r*vol) (677-681)
-        double num[NF * D];
-        bool shared = div_operand_ok(rv) & (rv > 0.0);
-#pragma unroll
-        for (int n = 0; n < NF; n++) {
-            const double pj = r[D + n];
-            pmax[n] = fmax(pmax[n], pj);
-            pmin[n] = fmin(pmin[n], pj);
-            const double d_dif = pj - pv[n];
-            const double d_sum = pj + pv[n];
+        for (int b = 0; b < GB; b++) {
+            if (cur[b].j == INVALID) continue;
+            const double* com = cur[b].com;
+            const double area = cur[b].area;
+            const double* r = rb[b];
+            double cfx[D], dr[D], rr = 0.0;
 #pragma unroll
             for (int k = 0; k < D; k++) {
-                num[n * D + k] = area * (d_dif * cfx[k] - 0.5 * d_sum * dr[k]);
-                shared &= div_operand_ok(num[n * D + k]);
+                const double xj = r[k];
+                cfx[k] = com[k] - 0.5 * (xi[k] + xj);
+                dr[k] = xi[k] - xj;
+                rr += dr[k] * dr[k];
             }
-        }
-        if (shared) {
-            const double rcp = rcp_newton(rv);
+            rr = sqrt(rr);
+            const double rv = rr * vol;
+            // Springel (2010) eq. 21 summand, area*(...)/(r*vol) (677-681)
+            double num[NF * D];
+            bool shared = div_operand_ok(rv) & (rv > 0.0);
 #pragma unroll
-            for (int q = 0; q < NF * D; q++) df[q] += div_shared(num[q], rv, rcp);
-        } else {
+            for (int n = 0; n < NF; n++) {
+                const double pj = r[D + n];
+                pmax[n] = fmax(pmax[n], pj);
+                pmin[n] = fmin(pmin[n], pj);
+                const double d_dif = pj - pv[n];
+                const double d_sum = pj + pv[n];
 #pragma unroll
-            for (int q = 0; q < NF * D; q++) df[q] += num[q] / rv;
+                for (int k = 0; k < D; k++) {
+                    num[n * D + k] = area * (d_dif * cfx[k] - 0.5 * d_sum * dr[k]);
+                    shared &= div_operand_ok(num[n * D + k]);
+                }
+            }
+            if (shared) {
+                const double rcp = rcp_newton(rv);
+#pragma unroll
+                for (int q = 0; q < NF * D; q++) df[q] += div_shared(num[q], rv, rcp);
+            } else {
+#pragma unroll
+                for (int q = 0; q < NF * D; q++) df[q] += num[q] / rv;
+            }
         }
     }
 
@@ -576,19 +593,27 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
         double dpos[NF], dneg[NF];
 #pragma unroll
         for (int n = 0; n < NF; n++) { dpos[n] = 0.0; dneg[n] = 0.0; }
-        for (int e = 0; e < deg; e++) {
-            const long long slot = base + (long long)e * SLICE;
-            double rel[D];
-            rel[0] = __builtin_nontemporal_load(sx + slot) - cx[0];
-            rel[1] = __builtin_nontemporal_load(sy + slot) - cx[1];
-            if (D == 3) rel[D - 1] = __builtin_nontemporal_load(sz + slot) - cx[D - 1];
+        for (int e0 = 0; e0 < deg; e0 += LB) {
+            double relb[LB][D];
 #pragma unroll
-            for (int n = 0; n < NF; n++) {
-                double dphi = 0.0;
+            for (int b = 0; b < LB; b++) {
+                if (e0 + b >= deg) continue;
+                const long long slot = base + (long long)(e0 + b) * SLICE;
+                relb[b][0] = __builtin_nontemporal_load(sx + slot) - cx[0];
+                relb[b][1] = __builtin_nontemporal_load(sy + slot) - cx[1];
+                if (D == 3) relb[b][D - 1] = __builtin_nontemporal_load(sz + slot) - cx[D - 1];
+            }
 #pragma unroll
-                for (int k = 0; k < D; k++) dphi += df[n * D + k] * rel[k];
-                if (dphi > 0) dpos[n] = fmax(dpos[n], dphi);
-                else if (dphi < 0) dneg[n] = fmin(dneg[n], dphi);
+            for (int b = 0; b < LB; b++) {
+                if (e0 + b >= deg) continue;
+#pragma unroll
+                for (int n = 0; n < NF; n++) {
+                    double dphi = 0.0;
+#pragma unroll
+                    for (int k = 0; k < D; k++) dphi += df[n * D + k] * relb[b][k];
+                    if (dphi > 0) dpos[n] = fmax(dpos[n], dphi);
+                    else if (dphi < 0) dneg[n] = fmin(dneg[n], dphi);
+                }
             }
         }
 #pragma unroll
