@@ -68,7 +68,16 @@ class StaticStepper(object):
         pc = self.p.particles
         self.prim_init = [pc[f][self.real].clone() for f in self.prim_names]
 
+    def lane(self):
+        """The stepper's second execution lane (created on first use)."""
+        if getattr(self, "_lane", None) is None:
+            self._lane = Lane(self)
+        return self._lane
+
     def close(self):
+        if getattr(self, "_lane", None) is not None:
+            self._lane.close()
+            self._lane = None
         if self.adj:
             self.lib.phd_adjoint_destroy(self.adj)
             self.adj = None
@@ -183,16 +192,65 @@ class StaticStepper(object):
         self.refresh()
 
 
+class Lane(object):
+    """A second execution lane of a StaticStepper: its own stream (default
+    priority), library handle, plan and cell buffers over the same mesh, so
+    that forward steps can run concurrently with the stepper's own work."""
+
+    def __init__(self, st):
+        self.st = st
+        pc = st.p.particles
+        dev = pc[st.cons_names[0]].device
+        self.stream = torch.cuda.Stream(device=dev)
+        hp = ctypes.c_void_p()
+        nat.check(st.lib.phd_handle_create(ctypes.byref(hp), dev.index), "phd_handle_create")
+        self.h = hp.value
+        st.lib.phd_set_stream(self.h, ctypes.c_void_p(self.stream.cuda_stream))
+        self.cons = [torch.empty_like(pc[f]) for f in st.cons_names]
+        self.prim = [torch.empty_like(pc[f]) for f in st.prim_names]
+        self.cells = st.view(self.cons, self.prim)
+        self.plan = nat.Plan(self.cells, st.faces)
+
+    def load(self, state):
+        """Cells of this lane = a full state (conserved + primitive)."""
+        for t, v in zip(self.cons + self.prim, state):
+            t.copy_(v)
+
+    def step(self, dt, rec, packed):
+        st = self.st
+        lo, hi = ctypes.cast(st.lo, ctypes.c_void_p), ctypes.cast(st.hi, ctypes.c_void_p)
+        rc = st.lib.phd_core_step(self.h, self.plan.ptr, ctypes.byref(self.cells),
+                                  ctypes.byref(st.faces), rec.data_ptr(), st.grad.ld,
+                                  nat.LIMITER_AREPO, st.boundary, lo, hi, st.gamma, float(dt),
+                                  st.boost, st.gizmo, nat.SOLVER_HLLC, None,
+                                  nat.CORE_XP_FRESH if packed else 0)
+        nat.check(rc, "phd_core_step")
+        nat.check(st.lib.phd_ghost_refresh_eos(self.h, self.plan.ptr, ctypes.byref(self.cells),
+                                               st.boundary, lo, hi, st.gamma),
+                  "phd_ghost_refresh_eos")
+
+    def close(self):
+        if self.h:
+            torch.cuda.current_stream().wait_stream(self.stream)
+            self.stream.synchronize()
+            self.plan.close()
+            self.st.lib.phd_handle_destroy(self.h)
+            self.h = None
+
+
 class Rollout(torch.autograd.Function):
     """Final real-cell density after n static-mesh steps, as a function of
     the initial real-cell density and one initial velocity component.
 
-    Backward, per segment [s0, s1): restore the checkpoint, replay steps
-    s0..s1-2 keeping each start state's conserved fields and gradient
-    records, run the VJP of step s1-1 on the state now in place, then for
-    each earlier step refresh a cells view over its kept conserved fields
-    (primitives, ghosts and packed rows are functions of them) and run its
-    VJP there on the kept gradients: no copies back."""
+    Backward, per segment [s0, s1), last segment first: a second lane
+    (stream, handle, plan, cells) restores the segment's checkpoint and
+    replays it, keeping each step's start conserved fields and the gradient
+    records steps s0..s1-2 wrote; then, on the caller's stream, each step in
+    reverse gets a refresh of a cells view over its kept conserved fields
+    (primitives, ghosts and packed rows are functions of them) and its VJP,
+    on the kept gradients (step s1-1: recomputed).  The replay of the next
+    segment runs on the lane while this segment's VJPs run: two buffer sets,
+    ordered by events."""
 
     @staticmethod
     def forward(ctx, rho0, v0, stepper, n_steps, every, axis, dts=None):
@@ -226,25 +284,46 @@ class Rollout(torch.autograd.Function):
         lam = st.eos_vjp(1, lam_prim)
         bufs = (lam, [torch.zeros_like(t) for t in lam])   # ghost entries stay zero
         nseg = min(ctx.every, ctx.n)
-        snaps = [[torch.empty_like(pc[f]) for f in st.cons_names] for _ in range(nseg)]
+        lane = st.lane()
+        main = torch.cuda.current_stream()
         prim_tmp = [torch.empty_like(pc[f]) for f in st.prim_names]
-        views = [st.view(sn, prim_tmp) for sn in snaps]
-        recs = [st.new_rec() for _ in range(nseg)]
-        starts = sorted(ctx.checkpoints)
-        for s0 in reversed(starts):
-            s1 = min(s0 + ctx.every, ctx.n)
-            st.set_state(ctx.checkpoints[s0])
-            for n in range(s0, s1 - 1):
-                st.save_cons(snaps[n - s0])
-                st.step(ctx.dts[n], rec=recs[n - s0])
+        sets = []
+        for _ in range(2):
+            snaps = [[torch.empty_like(pc[f]) for f in st.cons_names] for _ in range(nseg)]
+            sets.append({"snaps": snaps, "views": [st.view(sn, prim_tmp) for sn in snaps],
+                         "recs": [st.new_rec() for _ in range(nseg)],
+                         "ready": torch.cuda.Event(), "free": torch.cuda.Event()})
+        for b in sets:
+            b["free"].record(main)
+        segs = [(s0, min(s0 + ctx.every, ctx.n)) for s0 in reversed(sorted(ctx.checkpoints))]
+
+        def replay(k):
+            s0, s1 = segs[k]
+            b = sets[k % 2]
+            with torch.cuda.stream(lane.stream):
+                lane.stream.wait_event(b["free"])
+                lane.load(ctx.checkpoints[s0])
+                for n in range(s0, s1):
+                    for t, u in zip(b["snaps"][n - s0], lane.cons):
+                        t.copy_(u)
+                    if n < s1 - 1:
+                        lane.step(ctx.dts[n], b["recs"][n - s0], packed=n > s0)
+                b["ready"].record(lane.stream)
+
+        replay(0)
+        for k, (s0, s1) in enumerate(segs):
+            if k + 1 < len(segs):
+                replay(k + 1)
+            b = sets[k % 2]
+            main.wait_event(b["ready"])
             for n in reversed(range(s0, s1)):
                 out = bufs[1] if lam is bufs[0] else bufs[0]
-                if n == s1 - 1:
-                    lam = st.vjp(ctx.dts[n], lam, out=out)
-                else:
-                    st.refresh(views[n - s0])
-                    lam = st.vjp(ctx.dts[n], lam, rec=recs[n - s0], out=out,
-                                 cells=views[n - s0])
+                view = b["views"][n - s0]
+                st.refresh(view)
+                lam = st.vjp(ctx.dts[n], lam, rec=b["recs"][n - s0] if n < s1 - 1 else None,
+                             out=out, cells=view)
+            b["free"].record(main)
+        main.wait_stream(lane.stream)
         # initial conserved fields = cons(prim_0)
         st.set_state(ctx.checkpoints[0])
         lam_p0 = st.eos_vjp(0, lam)
