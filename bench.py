@@ -32,6 +32,9 @@ for p in (os.path.join(ROOT, "phd-code_amd"), ROOT):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+# FP64 VALU issue rate: a wave64 FP64 VALU instruction occupies a SIMD for 4
+# cycles; 256 CUs x 4 SIMDs at the 2.4 GHz peak engine clock
+VALU_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 CSRC = os.path.join(ROOT, "phd-code_amd", "phd", "csrc")
 PROFILES = os.path.join(ROOT, "profiles")
 # bench stage -> rocprofv3 kernel name of the 3D kernel it launches
@@ -104,7 +107,7 @@ def _pmc_means(csv_path, counter):
     return {k: sum(v) / len(v) for k, v in vals.items()}
 
 
-def write_traffic(fetch_csv, write_csv, n, command):
+def write_traffic(fetch_csv, write_csv, n, command, sq_csv=None):
     """Turn two rocprofv3 --pmc runs of this benchmark (FETCH_SIZE, then
     WRITE_SIZE; KiB per dispatch) into HBM bytes per launch per kernel.
     Each counter is scaled by what it reads for k_pack_xp, a copy kernel of
@@ -112,6 +115,7 @@ def write_traffic(fetch_csv, write_csv, n, command):
     per cell, all cells."""
     fetch = _pmc_means(fetch_csv, "FETCH_SIZE")
     write = _pmc_means(write_csv, "WRITE_SIZE")
+    valu = _pmc_means(sq_csv, "SQ_INSTS_VALU") if sq_csv else {}
     from phd.mesh import build_voronoi
     from phd.problems import jittered_lattice
     g = build_voronoi(3, jittered_lattice(n, 3, 0), [0.0] * 3, [1.0] * 3, 0)
@@ -125,6 +129,8 @@ def write_traffic(fetch_csv, write_csv, n, command):
             wr = write[kn] * 1024.0 * w_cal
             kernels[stage] = {"kernel": kn, "fetch_kib": fetch[kn], "write_kib": write[kn],
                               "read_bytes": rd, "write_bytes": wr, "traffic_bytes": rd + wr}
+            if kn in valu:
+                kernels[stage]["valu_wave_instr"] = valu[kn]
     rec = {"workload": "sedov3d n=%d" % n, "source_sha256": source_hash(),
            "copy_kernel": COPY_KERNEL, "copy_known_bytes": na_bytes,
            "fetch_scale": f_cal, "write_scale": w_cal, "kernels": kernels,
@@ -146,6 +152,39 @@ def measured_traffic(n, stage):
     if rec.get("source_sha256") != source_hash() or stage not in rec.get("kernels", {}):
         return None
     return rec["kernels"][stage]["traffic_bytes"]
+
+
+def measured_valu(n, stage):
+    """Wave-level FP64 VALU instructions per launch of a stage's kernel from
+    the committed counter record (SQ_INSTS_VALU), or None."""
+    try:
+        with open(traffic_path(n)) as fh:
+            rec = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if rec.get("source_sha256") != source_hash():
+        return None
+    return rec.get("kernels", {}).get(stage, {}).get("valu_wave_instr")
+
+
+def stream_copy_gbs(nbytes=2 << 30, reps=10):
+    """Device-to-device copy bandwidth measured on this GPU (read + write
+    bytes / time, HIP events), the practical HBM ceiling beside the 8 TB/s
+    peak."""
+    import torch
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
 def cpu_baseline(problem, dt):
@@ -302,6 +341,8 @@ def main():
     ap.add_argument("--workload", choices=("sedov", "ensemble"), default="sedov",
                     help="ensemble: only the batched-ensemble measurement, as its own line")
     ap.add_argument("--problems", type=int, default=4096)
+    ap.add_argument("--sq-from", default=None, metavar="SQ_CSV",
+                    help="with --traffic-from: a third rocprofv3 --pmc run with SQ_INSTS_VALU")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write the traffic record from two rocprofv3 --pmc runs of this "
                          "benchmark (FETCH_SIZE, WRITE_SIZE) and exit; no GPU needed")
@@ -310,7 +351,7 @@ def main():
     args = ap.parse_args()
     if args.traffic_from:
         rec = write_traffic(args.traffic_from[0], args.traffic_from[1], args.n,
-                            args.traffic_command)
+                            args.traffic_command, args.sq_from)
         print(json.dumps(rec, indent=1))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -424,6 +465,14 @@ def main():
                 "traffic": None if traffic is None else round(traffic),
                 "kernel": dom, "algorithmic_bytes": B[dom]}
 
+    vi = measured_valu(args.n, dom)
+    valu_line = None
+    if vi is not None:
+        floor_ms = vi / VALU_WAVE_INSTR_PER_S * 1e3
+        valu_line = {"kernel": dom, "wave_instr_per_launch": vi, "floor_ms": round(floor_ms, 4),
+                     "frac": round(floor_ms / kern_ms[dom], 4),
+                     "note": "FP64 VALU issue utilisation: instructions x 4 cycles over "
+                             "256 CUs x 4 SIMDs at 2.4 GHz, against the kernel's time"}
     result = {
         "metric": "Voronoi face-flux updates/s FP64 (3D Sedov 2M cells) + %HBM roofline; "
                   "adjoint/fwd",
@@ -448,6 +497,8 @@ def main():
         "core_face_updates_per_s": nf / (core_ms * 1e-3),
         "core_model_frac_hbm": round(B["core_model"] / (core_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,
+        "valu": valu_line,
+        "stream_copy_gbs": round(stream_copy_gbs(), 1),
         "cpu_baseline": None,
         "adjoint": None,
     }
