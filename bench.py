@@ -204,8 +204,8 @@ def cpu_baseline(problem, dt):
     return {"value": a["nf"] / t, "unit": "face-updates/s", "cores": 1, "kind": "port",
             "sample": "median of %d full core steps (gradients+limiter, ghost gradients, "
                       "spatial+temporal states, HLLC+deboost, update) of the same Sedov mesh, "
-                      "%d faces: %s s each on one core" %
-                      (len(times), a["nf"], ", ".join("%.2f" % x for x in times))}
+                      "%d faces: %s s each on one core of %s" %
+                      (len(times), a["nf"], ", ".join("%.2f" % x for x in times), cpu_model())}
 
 
 def adjoint_timing(n=256, n_steps=50, every=10, reps=3):
@@ -328,6 +328,40 @@ def ensemble_timing(n_problems=4096, steps=100, per_problem_steps=3, seed=0, log
     views = None
     ens.close()
     return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_omp(problem, dt):
+    """The same core step through the OpenMP build of the oracle (its
+    independent per-cell and per-face loops on all threads OpenMP is given;
+    the update stays serial, in face order): the all-cores CPU line."""
+    from phd import problems
+    from oracle import oracle
+    a = problems.host_arrays(problem)
+    threads = oracle.omp_threads()
+    times = []
+    while len(times) < 5 and sum(times) < 20.0:
+        t0 = time.perf_counter()
+        r = oracle.core_step(a, dt, omp=True)
+        times.append(time.perf_counter() - t0)
+        if r["rc"] != 0:
+            raise RuntimeError("oracle core step reported fault %d" % r["rc"])
+    t = sorted(times)[len(times) // 2]
+    return {"value": a["nf"] / t, "unit": "face-updates/s", "cores": threads, "kind": "port",
+            "sample": "median of %d full core steps of the same Sedov mesh through the OpenMP "
+                      "build of the oracle, %d threads on %s (%d logical CPUs): %s s each" %
+                      (len(times), threads, cpu_model(), os.cpu_count() or 0,
+                       ", ".join("%.2f" % x for x in times))}
 
 
 def main():
@@ -500,6 +534,7 @@ def main():
         "valu": valu_line,
         "stream_copy_gbs": round(stream_copy_gbs(), 1),
         "cpu_baseline": None,
+        "cpu_baseline_omp": None,
         "adjoint": None,
     }
     if not args.no_adjoint:
@@ -507,6 +542,7 @@ def main():
         result["adjoint_over_fwd"] = result["adjoint"]["ratio"]
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(prob, dt)
+        result["cpu_baseline_omp"] = cpu_baseline_omp(prob, dt)
     print(json.dumps(result), flush=True)
 
 

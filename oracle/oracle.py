@@ -12,8 +12,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libphd_oracle.so")
+OMP_PATH = os.path.join(HERE, "libphd_oracle_omp.so")
 
 _lib = None
+_omp = None
 dp = ctypes.POINTER(ctypes.c_double)
 
 
@@ -22,14 +24,27 @@ def build():
     return LIB_PATH
 
 
-def lib():
-    global _lib
+def lib(omp=False):
+    """The serial oracle, or (omp=True) the same source with its independent
+    loops run by OpenMP threads (same bits)."""
+    global _lib, _omp
+    if omp:
+        if _omp is None:
+            if not os.path.exists(OMP_PATH):
+                build()
+            _omp = ctypes.CDLL(OMP_PATH)
+            _omp.orc_compute_time_step.restype = ctypes.c_double
+        return _omp
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
         _lib = ctypes.CDLL(LIB_PATH)
         _lib.orc_compute_time_step.restype = ctypes.c_double
     return _lib
+
+
+def omp_threads():
+    return int(lib(omp=True).orc_threads())
 
 
 def _pp(arrs):
@@ -173,7 +188,7 @@ def update_from_fluxes(a, flux, dt, cons=None):
     return cons, rc, bad.value
 
 
-def core_step(a, dt, solver=1, boost=True, gizmo=True, limiter=0):
+def core_step(a, dt, solver=1, boost=True, gizmo=True, limiter=0, omp=False):
     """Full core step on copies; returns dict(grad, left, right, flux, cons, rc, bad)."""
     dim, n, nf = a["dim"], a["n"], a["nf"]
     grad = [np.zeros(n) for _ in range(dim * (dim + 2))]
@@ -184,7 +199,7 @@ def core_step(a, dt, solver=1, boost=True, gizmo=True, limiter=0):
     lo = np.array(a["lo"] + [0.0] * (3 - dim))
     hi = np.array(a["hi"] + [0.0] * (3 - dim))
     bad = ctypes.c_int32(-1)
-    rc = lib().orc_core_step(dim, n, nf, ctypes.c_double(a["gamma"]), ctypes.c_double(dt),
+    rc = lib(omp).orc_core_step(dim, n, nf, ctypes.c_double(a["gamma"]), ctypes.c_double(dt),
                              int(solver), int(boost), int(gizmo), int(limiter),
                              int(a["boundary"]), _p(a["tag"]), _p(a["type"]), _p(a["map"]),
                              _pp(a["x"]), _pp(a["dcom"]), _p(a["volume"]), _pp(a["prim"]),

@@ -49,6 +49,27 @@
 
 static const double PI = 3.141592653589793; /* numpy.pi */
 
+/* Loops whose iterations are independent (per cell or per face) carry
+ * ORC_PFOR: with -fopenmp (libphd_oracle_omp.so, the all-cores CPU line)
+ * they run in parallel, each iteration doing exactly the serial work, so
+ * both builds give the same bits; without it the macro is empty. */
+#define ORC_STR(x) #x
+#ifdef _OPENMP
+#include <omp.h>
+#define ORC_PFOR(clauses) _Pragma(ORC_STR(omp parallel for schedule(static) clauses))
+#else
+#define ORC_PFOR(clauses)
+#endif
+
+int orc_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 /* ------------------------------------------------------------------------ */
 /* equation of state  (equation_state.pyx:119-121)                          */
 static double sound_speed(double gamma, double d, double p)
@@ -189,6 +210,20 @@ int orc_compute_gradients(int dim, int n, const int8_t* tag, double* const* x,
     double phi_max[5], phi_min[5], alpha[5], df[15];
     double xi[3], xj[3], cx[3], cfx[3], dr[3];
 
+    /* the neighbour check (reconstruction.pyx:647-652): the first real cell
+     * in loop order with a face that is not its own raises */
+    for (int i = 0; i < n; i++) {
+        if (tag[i] != REAL) continue;
+        for (int m = nbr_off[i]; m < nbr_off[i+1]; m++) {
+            int fid = nbr_face[m];
+            if (i != pair_i[fid] && i != pair_j[fid]) {
+                if (bad_cell) *bad_cell = i;
+                return ORC_ERR_NEIGHBORS;
+            }
+        }
+    }
+
+    ORC_PFOR(private(phi_max, phi_min, alpha, df, xi, xj, cx, cfx, dr))
     for (int i = 0; i < n; i++) {
         if (tag[i] != REAL) continue;
         for (int k = 0; k < dim; k++) {
@@ -209,8 +244,7 @@ int orc_compute_gradients(int dim, int n, const int8_t* tag, double* const* x,
             int fid = nbr_face[m];
             double area = face_area[fid];
             if (i == pair_i[fid]) j = pair_j[fid];
-            else if (i == pair_j[fid]) j = pair_i[fid];
-            else { if (bad_cell) *bad_cell = i; return ORC_ERR_NEIGHBORS; }
+            else j = pair_i[fid];      /* i == pair_j[fid]: checked above */
 
             double r = 0.0;
             for (int k = 0; k < dim; k++) {
@@ -285,6 +319,7 @@ void orc_ghost_gradients(int dim, int n, const int8_t* tag, const int8_t* type,
                          double* const* grad)
 {
     const int ng = dim*(dim + 2);
+    ORC_PFOR()
     for (int i = 0; i < n; i++)
         if (tag[i] == GHOST)
             for (int g = 0; g < ng; g++)
@@ -335,6 +370,7 @@ void orc_add_spatial(int dim, int nf, double* const* prim, double* const* x,
     const double psi1 = 0.5, psi2 = 0.25;
     double state_l[5], state_r[5];
 
+    ORC_PFOR(private(state_l, state_r))
     for (int m = 0; m < nf; m++) {
         int i = pair_i[m], j = pair_j[m];
         for (int f = 0; f < nfld; f++) {
@@ -418,6 +454,7 @@ void orc_add_temporal(int dim, int nf, double gamma, double dt,
     double* const* dv = grad + dim;
     double* const* dp = grad + (dim+1)*dim;
 
+    ORC_PFOR(private(state_l, state_r, vi, vj))
     for (int m = 0; m < nf; m++) {
         int i = pair_i[m], j = pair_j[m];
         for (int f = 0; f < nfld; f++) {
@@ -556,6 +593,7 @@ void orc_hllc(int dim, int nf, double gamma, int boost, double* const* sL,
               double* const* sR, double* const* nx, double* const* fv,
               double* const* flux)
 {
+    ORC_PFOR()
     for (int i = 0; i < nf; i++) {
         double _dl = sL[0][i], _pl = sL[dim+1][i];
         double _dr = sR[0][i], _pr = sR[dim+1][i];

@@ -99,3 +99,19 @@ def test_dual_tangent_is_linear():
     for q in range(4):
         ref = 2.0 * d1[q] - 0.5 * d2[q]
         assert np.allclose(d3[q], ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("n,dim,boundary,seed", CASES)
+def test_openmp_build_gives_the_serial_bits(n, dim, boundary, seed):
+    """libphd_oracle_omp.so (the all-cores CPU line) runs the independent
+    loops in parallel; every output equals the serial oracle's bit for bit."""
+    from oracle import oracle
+    a, prim0 = _problem(n, dim, boundary, seed)
+    a = dict(a, prim=prim0, cons=oracle.cons_from_prim(dim, a["gamma"], prim0, a["volume"]))
+    r1 = oracle.core_step(a, 2e-3)
+    r2 = oracle.core_step(a, 2e-3, omp=True)
+    assert r1["rc"] == r2["rc"] == 0
+    for key in ("grad", "left", "right", "flux", "cons"):
+        for q, (x, y) in enumerate(zip(r1[key], r2[key])):
+            assert np.array_equal(x, y), (key, q)
+    assert oracle.omp_threads() >= 1
