@@ -982,11 +982,33 @@ __global__ __launch_bounds__(BS) void k_update_sell(phd_cells c,
 // pattern: exact) and flags bad[p] when the cell's mass, energy, density or
 // pressure is not positive; a wave reduces per problem first, one atomic
 // per problem it touches.
+// problem of cell i, searching forward from p0 (a problem at or before i's)
+__device__ __forceinline__ int problem_from(int p0, int i, const int32_t* __restrict__ cs,
+                                            int n_problems)
+{
+    int p = p0;
+    while (p + 1 < n_problems && cs[p + 1] <= i) p++;
+    return p;
+}
+
+// problem of cell i by binary search
+__device__ __forceinline__ int problem_of(int i, const int32_t* __restrict__ cs, int n_problems)
+{
+    int lo = 0, hi = n_problems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cs[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// p0: the problem of the block's first cell (problems own contiguous cell
+// ranges, so a lane's problem is found by stepping forward from it)
 template <int D>
 __device__ __forceinline__ void fold_cfl(bool real, int i, const double* u, const double* p,
                                          double vol, double gamma, int boost,
                                          const int32_t* __restrict__ cs, int n_problems,
-                                         unsigned long long* dtbits, int32_t* bad)
+                                         int p0, unsigned long long* dtbits, int32_t* bad)
 {
     double v = __longlong_as_double((long long)INF_BITS);
     int b = 0, pid = -1;
@@ -1003,12 +1025,7 @@ __device__ __forceinline__ void fold_cfl(bool real, int i, const double* u, cons
         }
         if (!(v >= 0.0)) v = __longlong_as_double((long long)INF_BITS);
         b = !(u[0] > 0.0) || !(u[D + 1] > 0.0) || !(p[0] > 0.0) || !(p[D + 1] > 0.0);
-        int lo = 0, hi = n_problems - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (cs[mid] <= i) lo = mid; else hi = mid - 1;
-        }
-        pid = lo;
+        pid = problem_from(p0, i, cs, n_problems);
     }
     // distinct problems of the wave, smallest first
     int todo = pid;
@@ -1046,6 +1063,12 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_ghost_refresh_eos(phd_cells c, in
     const long long ii = xcd_index(c.n_cells);
     const bool in = ii < c.n_cells;      // no early exit: the BATCH fold is wave-wide
     const int i = in ? (int)ii : 0;
+    __shared__ int s_p0;
+    if (BATCH) {
+        const long long w0 = ii - threadIdx.x;     // the block's first cell
+        if (threadIdx.x == 0) s_p0 = w0 < c.n_cells ? problem_of((int)w0, bcs, n_problems) : 0;
+        __syncthreads();
+    }
     double u[NF], p[NF], vol = 0.0;
     bool real = false;
     if (in) {
@@ -1091,7 +1114,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_ghost_refresh_eos(phd_cells c, in
             for (int q = 0; q < XPW / 2; q++) o[q] = make_double2(r[2 * q], r[2 * q + 1]);
         }
     }
-    if (BATCH) fold_cfl<D>(real, i, u, p, vol, gamma, boost, bcs, n_problems, dtbits, bad);
+    if (BATCH) fold_cfl<D>(real, i, u, p, vol, gamma, boost, bcs, n_problems, s_p0, dtbits, bad);
 }
 
 // ---------------------------------------------------------------------------
