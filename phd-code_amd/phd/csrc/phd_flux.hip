@@ -31,8 +31,8 @@
 //
 // Arithmetic: FP64 VALU only, compiled with -ffp-contract=off; every
 // expression follows the reference's order, so results are bit-identical to
-// its x86-64 build except where libm pow() and the GPU's pow() round
-// differently.
+// its x86-64 build except where libm pow() and the GPU's pow() (or, for bases
+// within 2^-20 of one, its binomial series) round differently.
 #include <string.h>
 #include "flux_common.h"
 

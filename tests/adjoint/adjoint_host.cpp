@@ -160,4 +160,10 @@ int ah_grad_cell(int dim, int deg, const double* rows, const double* com, const 
     return 0;
 }
 
+// pow_b1 over n (base, exponent) pairs, for its check against libm pow
+void ah_pow_b1(long n, const double* x, const double* y, double* out)
+{
+    for (long i = 0; i < n; i++) out[i] = pow_b1(x[i], y[i]);
+}
+
 }

@@ -292,3 +292,40 @@ def test_grad_cell_vjp(lib, dim):
         return np.concatenate([pv, nb.ravel()])
 
     _run(300, f, vjp, G, gen, 13)
+
+
+def test_pow_near_one(lib):
+    """pow_b1 (riemann_dev.h): the quadratic binomial series inside
+    |x - 1|(|y| + 3) <= 2^-20 and libm pow outside it, against x86 long-double
+    powl (11 more bits).  Exponents are the ones the solvers use, for gammas
+    from 1.001 to 3; bases one ulp to the guard's edge away from 1 and beyond.
+    Every result is faithful (within one ulp of the true power) and within one
+    ulp of libm pow; x == 1 gives exactly 1.  A dropped or mis-signed quadratic
+    term misses by ~1e3 ulp at the guard's edge."""
+    rng = np.random.default_rng(11)
+    ys = []
+    for g in (1.001, 1.1, 1.4, 5.0 / 3.0, 3.0):
+        z = (g - 1.0) / (2.0 * g)
+        ys += [z, 1.0 / z, -(g + 1.0) / (2.0 * g), 1.0 / g, 2.0 / (g - 1.0), 2.0 * g / (g - 1.0)]
+    xs, yv = [], []
+    for y in ys:
+        edge = 2.0 ** -20 / (abs(y) + 3.0)
+        d = np.concatenate([np.arange(-64, 65) * 2.0 ** -52,
+                            rng.uniform(-edge, edge, 400),
+                            edge * np.array([-1.0, 1.0, 0.999, -0.999, 1.001, -1.001]),
+                            rng.uniform(-0.3, 0.3, 100)])
+        xs.append(1.0 + d)
+        yv.append(np.full(d.size, y))
+    x = np.concatenate(xs)
+    y = np.concatenate(yv)
+    out = np.empty_like(x)
+    lib.ah_pow_b1(ctypes.c_long(x.size), _p(x), _p(y), _p(out))
+    truth = np.power(x.astype(np.longdouble), y.astype(np.longdouble))
+    ulp = np.spacing(np.abs(out))
+    err = np.abs(out.astype(np.longdouble) - truth) / ulp.astype(np.longdouble)
+    assert float(err.max()) <= 1.0, float(err.max())
+    libm = np.power(x, y)
+    assert np.all(np.abs(out - libm) <= ulp)
+    assert np.all(out[x == 1.0] == 1.0)
+    inside = np.abs(x - 1.0) * (np.abs(y) + 3.0) <= 2.0 ** -20
+    assert inside.sum() > 0.8 * x.size and (~inside).sum() >= 100

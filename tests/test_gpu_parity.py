@@ -4,7 +4,7 @@ ABI, on the three synthetic configurations at oracle-sized meshes.
 Each stage is fed the same inputs on both sides (the oracle gets host copies
 of the device arrays) and compared with the tolerance of tests/parity.py;
 BITWISE kernels are expected to agree exactly except where libm pow() and
-the GPU's pow() round differently.
+the GPU's pow() (or its series for bases near one) round differently.
 """
 import numpy as np
 import pytest
