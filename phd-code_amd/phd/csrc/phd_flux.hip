@@ -34,6 +34,7 @@
 // its x86-64 build except where libm pow() and the GPU's pow() (or, for bases
 // within 2^-20 of one, its binomial series) round differently.
 #include <string.h>
+#include <vector>
 #include "flux_common.h"
 
 using namespace phd;
@@ -901,25 +902,22 @@ __device__ __forceinline__ void load_summand(const double* g, int fid, double* s
     }
 }
 
-template <int D, int BS = PHD_BLOCK, int XCD = 1>
-__global__ __launch_bounds__(BS) void k_update_sell(phd_cells c,
-                                                           const double* __restrict__ g,
-                                                           int owner_runs,
-                                                           const int2* __restrict__ csr,
-                                                           const int32_t* __restrict__ off,
-                                                           const int32_t* __restrict__ perm,
-                                                           const int32_t* __restrict__ sbase,
-                                                           const int32_t* __restrict__ sfi,
-                                                           const int2* __restrict__ cinfo,
-                                                           unsigned long long* err)
+// Conserved update of the cell at sliced position t (cell i = perm[t]):
+// u = its conserved values plus its summand rows in list order.  False for a
+// ghost (nothing to write; a ghost with faces reports a fault).
+template <int D>
+__device__ __forceinline__ bool update_cell(const phd_cells& c, long long t, int i,
+                                            const double* __restrict__ g, int owner_runs,
+                                            const int2* __restrict__ csr,
+                                            const int32_t* __restrict__ off,
+                                            const int32_t* __restrict__ sbase,
+                                            const int32_t* __restrict__ sfi,
+                                            const int2* __restrict__ cinfo,
+                                            unsigned long long* err, double* u)
 {
     constexpr int NF = D + 2, GW = summand_width<D>(), B = 4;
-    const long long t = sched_index<BS, XCD>(c.n_cells);
-    if (t >= c.n_cells) return;
-    const int i = perm[t];
     const int beg = off[i], end = off[i + 1];
-    if (ghost_fault<D>(c, i, beg, end, csr, err)) return;
-    double u[NF];
+    if (ghost_fault<D>(c, i, beg, end, csr, err)) return false;
 #pragma unroll
     for (int q = 0; q < NF; q++) u[q] = c.cons[q][i];
     const int deg = end - beg;
@@ -964,6 +962,27 @@ __global__ __launch_bounds__(BS) void k_update_sell(phd_cells c,
             if (apply_summand<D>(u, s, en & INCOMING, fid, err)) break;
         }
     }
+    return true;
+}
+
+template <int D, int BS = PHD_BLOCK, int XCD = 1>
+__global__ __launch_bounds__(BS) void k_update_sell(phd_cells c,
+                                                           const double* __restrict__ g,
+                                                           int owner_runs,
+                                                           const int2* __restrict__ csr,
+                                                           const int32_t* __restrict__ off,
+                                                           const int32_t* __restrict__ perm,
+                                                           const int32_t* __restrict__ sbase,
+                                                           const int32_t* __restrict__ sfi,
+                                                           const int2* __restrict__ cinfo,
+                                                           unsigned long long* err)
+{
+    constexpr int NF = D + 2;
+    const long long t = sched_index<BS, XCD>(c.n_cells);
+    if (t >= c.n_cells) return;
+    const int i = perm[t];
+    double u[NF];
+    if (!update_cell<D>(c, t, i, g, owner_runs, csr, off, sbase, sfi, cinfo, err, u)) return;
 #pragma unroll
     for (int q = 0; q < NF; q++) c.cons[q][i] = u[q];
 }
@@ -982,6 +1001,37 @@ __global__ __launch_bounds__(BS) void k_update_sell(phd_cells c,
 // pattern: exact) and flags bad[p] when the cell's mass, energy, density or
 // pressure is not positive; a wave reduces per problem first, one atomic
 // per problem it touches.
+// IdealGas.primitive_from_conservative (equation_state.pyx:104-117) for one
+// cell, in the reference's operation order
+template <int D>
+__device__ __forceinline__ void eos_prim(const double* u, double vol, double gamma, double* p)
+{
+    p[0] = u[0] / vol;
+    double v_sq = 0.;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        p[1 + k] = u[1 + k] / u[0];
+        v_sq += p[1 + k] * p[1 + k];
+    }
+    p[D + 1] = (u[D + 1] / vol - .5 * p[0] * v_sq) * (gamma - 1.);
+}
+
+// the primitive part of cell i's packed [position | primitive] row (the
+// positions of a static mesh stay as written)
+template <int D>
+__device__ __forceinline__ void store_xp_prim(double* xp, long long i, const double* p)
+{
+    double* r = xp + (size_t)i * XPW;
+    if (D == 2) {
+        reinterpret_cast<double2*>(r)[1] = make_double2(p[0], p[1]);
+        reinterpret_cast<double2*>(r)[2] = make_double2(p[2], p[3]);
+    } else {
+        r[3] = p[0];
+        reinterpret_cast<double2*>(r)[2] = make_double2(p[1], p[2]);
+        reinterpret_cast<double2*>(r)[3] = make_double2(p[3], p[D + 1]);
+    }
+}
+
 // problem of cell i, searching forward from p0 (a problem at or before i's)
 __device__ __forceinline__ int problem_from(int p0, int i, const int32_t* __restrict__ cs,
                                             int n_problems)
@@ -1048,6 +1098,31 @@ __device__ __forceinline__ void fold_cfl(bool real, int i, const double* u, cons
     }
 }
 
+// conserved values of ghost i from its image, momenta negated on each axis
+// where a reflective exterior ghost lies outside the box (domain_manager.pyx:
+// 759-774, boundary.pyx:336-348); stored into the ghost and returned in u
+template <int D>
+__device__ __forceinline__ void ghost_cons(const phd_cells& c, int i, int reflect, double lo0,
+                                           double lo1, double lo2, double hi0, double hi1,
+                                           double hi2, double* u)
+{
+    constexpr int NF = D + 2;
+    const int s = c.map[i];
+#pragma unroll
+    for (int q = 0; q < NF; q++) u[q] = c.cons[q][s];
+    if (reflect && c.type[i] == 2) {
+        const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            double xk = c.x[k][i];
+            if (xk < lo[k]) u[1 + k] *= -1;
+            if (xk > hi[k]) u[1 + k] *= -1;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NF; q++) c.cons[q][i] = u[q];
+}
+
 template <int D, int BATCH = 0>
 __global__ __launch_bounds__(PHD_BLOCK) void k_ghost_refresh_eos(phd_cells c, int reflect,
                                                                  double lo0, double lo1,
@@ -1073,34 +1148,14 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_ghost_refresh_eos(phd_cells c, in
     bool real = false;
     if (in) {
         if (c.tag[i] == 1) {
-            const int s = c.map[i];
-#pragma unroll
-            for (int q = 0; q < NF; q++) u[q] = c.cons[q][s];
-            if (reflect && c.type[i] == 2) {
-                const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
-#pragma unroll
-                for (int k = 0; k < D; k++) {
-                    double xk = c.x[k][i];
-                    if (xk < lo[k]) u[1 + k] *= -1;
-                    if (xk > hi[k]) u[1 + k] *= -1;
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < NF; q++) c.cons[q][i] = u[q];
+            ghost_cons<D>(c, i, reflect, lo0, lo1, lo2, hi0, hi1, hi2, u);
         } else {
             real = true;
 #pragma unroll
             for (int q = 0; q < NF; q++) u[q] = c.cons[q][i];
         }
         vol = c.volume[i];
-        p[0] = u[0] / vol;
-        double v_sq = 0.;
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            p[1 + k] = u[1 + k] / u[0];
-            v_sq += p[1 + k] * p[1 + k];
-        }
-        p[D + 1] = (u[D + 1] / vol - .5 * p[0] * v_sq) * (gamma - 1.);
+        eos_prim<D>(u, vol, gamma, p);
 #pragma unroll
         for (int q = 0; q < NF; q++) c.prim[q][i] = p[q];
         if (xp) {
@@ -1115,6 +1170,94 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_ghost_refresh_eos(phd_cells c, in
         }
     }
     if (BATCH) fold_cfl<D>(real, i, u, p, vol, gamma, boost, bcs, n_problems, s_p0, dtbits, bad);
+}
+
+// BATCH conserved update fused with the epilogue of its real cells: the
+// update of k_update_sell, then the primitives (equation_state.pyx:104-117),
+// the primitive part of the packed row and the per-problem CFL fold, as
+// k_ghost_refresh_eos<D, 1> does for real cells, with the same expressions.
+// A block of SORT_WINDOW threads owns a whole window (the degree sort
+// permutes within windows): the updated conserved values pass through LDS
+// and every field is then written in natural cell order, whole cache lines
+// per wave.  Ghosts are left to k_batch_ghosts, launched after this kernel
+// (their images are the updated real cells).
+template <int D, int BS = PHD_BLOCK, int XCD = 1>
+__global__ __launch_bounds__(BS) void k_update_eos(phd_cells c, const double* __restrict__ g,
+                                                   int owner_runs, const int2* __restrict__ csr,
+                                                   const int32_t* __restrict__ off,
+                                                   const int32_t* __restrict__ perm,
+                                                   const int32_t* __restrict__ sbase,
+                                                   const int32_t* __restrict__ sfi,
+                                                   const int2* __restrict__ cinfo,
+                                                   unsigned long long* err, double gamma,
+                                                   double* __restrict__ xp,
+                                                   const int32_t* __restrict__ bcs,
+                                                   int n_problems, int boost,
+                                                   unsigned long long* dtbits, int32_t* bad)
+{
+    constexpr int NF = D + 2;
+    constexpr bool STAGE = BS == SORT_WINDOW;
+    __shared__ double su[STAGE ? NF * SORT_WINDOW : 1];
+    __shared__ int s_p0;
+    const long long n = c.n_cells;
+    const long long t = sched_index<BS, XCD>(n);
+    const long long w0 = t - threadIdx.x;          // the block's first position
+    const long long ws = w0 - w0 % SORT_WINDOW;    // its window's first cell
+    if (threadIdx.x == 0) s_p0 = ws < n ? problem_of((int)ws, bcs, n_problems) : 0;
+    double u[NF], p[NF];
+    int i = t < n ? perm[t] : 0;
+    bool real = t < n && update_cell<D>(c, t, i, g, owner_runs, csr, off, sbase, sfi, cinfo, err, u);
+    if (STAGE) {
+        if (real) {
+#pragma unroll
+            for (int q = 0; q < NF; q++) su[q * SORT_WINDOW + (i - w0)] = u[q];
+        }
+        __syncthreads();
+        const long long k = w0 + threadIdx.x;
+        i = k < n ? (int)k : 0;
+        real = k < n && c.tag[i] == 0;
+        if (real) {
+#pragma unroll
+            for (int q = 0; q < NF; q++) u[q] = su[q * SORT_WINDOW + threadIdx.x];
+        }
+    } else {
+        __syncthreads();                           // s_p0
+    }
+    double vol = 0.0;
+    if (real) {
+        vol = c.volume[i];
+        eos_prim<D>(u, vol, gamma, p);
+#pragma unroll
+        for (int q = 0; q < NF; q++) {
+            c.cons[q][i] = u[q];
+            c.prim[q][i] = p[q];
+        }
+        store_xp_prim<D>(xp, i, p);
+    }
+    fold_cfl<D>(real, i, u, p, vol, gamma, boost, bcs, n_problems, s_p0, dtbits, bad);
+}
+
+// BATCH ghosts after k_update_eos: conserved values from their images
+// (ghost_cons), primitives and the primitive part of the packed row; gl
+// lists the ghost cells
+template <int D>
+__global__ __launch_bounds__(PHD_BLOCK) void k_batch_ghosts(phd_cells c,
+                                                            const int32_t* __restrict__ gl,
+                                                            int n_ghosts, int reflect,
+                                                            double lo0, double lo1, double lo2,
+                                                            double hi0, double hi1, double hi2,
+                                                            double gamma, double* __restrict__ xp)
+{
+    constexpr int NF = D + 2;
+    const long long k = xcd_index(n_ghosts);
+    if (k >= n_ghosts) return;
+    const int i = gl[k];
+    double u[NF], p[NF];
+    ghost_cons<D>(c, i, reflect, lo0, lo1, lo2, hi0, hi1, hi2, u);
+    eos_prim<D>(u, c.volume[i], gamma, p);
+#pragma unroll
+    for (int q = 0; q < NF; q++) c.prim[q][i] = p[q];
+    store_xp_prim<D>(xp, i, p);
 }
 
 // ---------------------------------------------------------------------------
@@ -1755,6 +1898,8 @@ struct phd_batch {
     unsigned long long* dtbits;  // per problem: CFL minimum folded by the refresh (bits)
     int32_t* badp;          // per problem: a non-positive state seen by the refresh
     int32_t* bad;           // creation check: faces joining two problems
+    int32_t* ghosts;        // the ghost cells (tag 1), ascending
+    int n_ghosts;
     hipStream_t stream;
     hipEvent_t ev_in, ev_out;
     hipGraph_t graph;
@@ -1902,6 +2047,35 @@ int batch_update(phd_handle* h, const phd_schedule& s, phd_plan* p, const phd_ce
 #undef X
     return PHD_E_ARG;
 }
+// the update fused with the real cells' epilogue, then the ghosts
+template <int D>
+int batch_update_eos(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c, int boundary,
+                     const double* lo, const double* hi, double gamma, int boost)
+{
+    const phd_schedule& s = b->sched;
+    const long long n = c->n_cells;
+    bool done = false;
+#define X(BS, XC)                                                                              \
+    if (!done && s.update_block == BS && s.xcd == XC) {                                        \
+        k_update_eos<D, BS, XC><<<sched_grid<BS, XC>(n), BS, 0, h->stream>>>(                  \
+            *c, p->g, p->owner_runs, p->csr, p->off, p->perm, p->sbase, p->sfi, p->cinfo,      \
+            h->d_err, gamma, p->xp, b->cell_start, b->n_problems, boost, b->dtbits, b->badp);  \
+        LAUNCHCHK();                                                                           \
+        done = true;                                                                           \
+    }
+    PHD_SCHED_CASES(X)
+#undef X
+    if (!done) return PHD_E_ARG;
+    if (b->n_ghosts > 0) {
+        double l[3], u[3];
+        bounds3(D, boundary, lo, hi, l, u);
+        k_batch_ghosts<D><<<xcd_grid(b->n_ghosts), PHD_BLOCK, 0, h->stream>>>(
+            *c, b->ghosts, b->n_ghosts, boundary == PHD_BOUNDARY_REFLECTIVE, l[0], l[1], l[2],
+            u[0], u[1], u[2], gamma, p->xp);
+        LAUNCHCHK();
+    }
+    return 0;
+}
 #undef PHD_SCHED_CASES
 
 bool sched_ok(const phd_schedule& s)
@@ -1948,8 +2122,7 @@ int batch_step(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c, con
     if (!rc) rc = batch_grad<D>(h, p, c, b->sched, rec, ld);
     if (!rc) rc = phd_update_ghost_gradients(h, p, c, boundary, lo, hi, rec, ld);
     if (!rc) rc = batch_face<D>(h, b, p, c, f, rec, ld, gamma, boost, gizmo);
-    if (!rc) rc = batch_update<D>(h, b->sched, p, c);
-    if (!rc) rc = batch_refresh<D>(h, b, p, c, boundary, lo, hi, gamma, boost);
+    if (!rc) rc = batch_update_eos<D>(h, b, p, c, boundary, lo, hi, gamma, boost);
     if (rc) return rc;
     k_batch_tick<<<1, 64, 0, h->stream>>>(b->counter);
     LAUNCHCHK();
@@ -1984,7 +2157,20 @@ int phd_batch_create(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_f
     if (cell_start[0] != 0 || cell_start[n_problems] != c->n_cells) return PHD_E_ARG;
     for (int q = 0; q < n_problems; q++)
         if (cell_start[q + 1] <= cell_start[q]) return PHD_E_ARG;
+    // the ghost list (tags are 0 real or 1 ghost)
+    std::vector<int8_t> tag((size_t)c->n_cells);
+    if (c->n_cells > 0 &&
+        (hipMemcpyAsync(tag.data(), c->tag, (size_t)c->n_cells, hipMemcpyDeviceToHost, h->stream) !=
+             hipSuccess ||
+         hipStreamSynchronize(h->stream) != hipSuccess))
+        return PHD_E_HIP;
+    std::vector<int32_t> gl;
+    for (int i = 0; i < c->n_cells; i++) {
+        if (tag[i] == 1) gl.push_back(i);
+        else if (tag[i] != 0) return PHD_E_ARG;
+    }
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t s_gl = al(4 * gl.size() + 4);
     const size_t s_cs = al(4 * (size_t)(n_problems + 1)), s_fp = al(4 * (size_t)f->n_faces + 4);
     const size_t s_dt = al(8 * (size_t)n_problems), s_h = al(8 * (size_t)hist_steps * (size_t)n_problems);
     const size_t s_ct = al(8), s_fa = al(4 * (size_t)n_problems), s_db = al(8 * (size_t)n_problems);
@@ -1995,7 +2181,8 @@ int phd_batch_create(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_f
     b->n_faces = f->n_faces;
     b->hist_cap = hist_steps;
     b->sched = phd_schedule{256, 256, 256, 1};
-    b->bytes = s_cs + s_fp + s_dt + s_h + 2 * s_ct + 2 * s_fa + s_db;
+    b->n_ghosts = (int)gl.size();
+    b->bytes = s_cs + s_fp + s_dt + s_h + 2 * s_ct + 2 * s_fa + s_db + s_gl;
     if (hipMalloc(&b->base, b->bytes) != hipSuccess) { delete b; return PHD_E_HIP; }
     char* q = (char*)b->base;
     b->cell_start = (int32_t*)q; q += s_cs;
@@ -2006,13 +2193,17 @@ int phd_batch_create(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_f
     b->fault = (int32_t*)q; q += s_fa;
     b->dtbits = (unsigned long long*)q; q += s_db;
     b->badp = (int32_t*)q; q += s_fa;
-    b->bad = (int32_t*)q;
+    b->bad = (int32_t*)q; q += s_ct;
+    b->ghosts = (int32_t*)q;
     int32_t hbad = 0;
     bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
     if (ok) ok = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) == hipSuccess;
     if (ok) ok = hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipMemcpyAsync(b->cell_start, cell_start, 4 * (size_t)(n_problems + 1),
                               hipMemcpyHostToDevice, h->stream) == hipSuccess;
+    if (!gl.empty())
+        ok = ok && hipMemcpyAsync(b->ghosts, gl.data(), 4 * gl.size(), hipMemcpyHostToDevice,
+                                  h->stream) == hipSuccess;
     if (ok) {
         k_batch_zero<<<1, 64, 0, h->stream>>>(b->counter);
         k_batch_zero<<<1, 64, 0, h->stream>>>(b->bad);
