@@ -192,8 +192,12 @@ struct CsrNbr {
     PHD_HD void row(int e, double* r) const
     {
         const int o = csr[beg + e].y & 0x7fffffff;
+        // as [position | primitive], the order grad_cell_vjp reads
+        const double* s = xp + (size_t)o * XPW;
 #pragma unroll
-        for (int q = 0; q < XPW; q++) r[q] = xp[(size_t)o * XPW + q];
+        for (int k = 0; k < D; k++) r[k] = s[XpLay<D>::X + k];
+#pragma unroll
+        for (int q = 0; q < D + 2; q++) r[D + q] = s[XpLay<D>::P + q];
     }
     PHD_HD void geo(int e, double* cm, double& a) const
     {
@@ -277,9 +281,9 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_adj_cell(phd_cells c, int n_real,
     }
     double pv[NF], xi[D], cx[D], vol;
 #pragma unroll
-    for (int q = 0; q < NF; q++) pv[q] = xp[(size_t)i * XPW + D + q];
+    for (int q = 0; q < NF; q++) pv[q] = xp[(size_t)i * XPW + XpLay<D>::P + q];
 #pragma unroll
-    for (int k = 0; k < D; k++) { xi[k] = xp[(size_t)i * XPW + k]; cx[k] = geo[(size_t)i * GEO + k]; }
+    for (int k = 0; k < D; k++) { xi[k] = xp[(size_t)i * XPW + XpLay<D>::X + k]; cx[k] = geo[(size_t)i * GEO + k]; }
     vol = geo[(size_t)i * GEO + 3];
     CsrNbr<D> nb{csr, off[i], off[i + 1] - off[i], xp, frec, contrib};
     grad_cell_vjp<D>(nb, pv, xi, cx, vol, s + NF, s);

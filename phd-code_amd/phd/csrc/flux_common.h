@@ -62,6 +62,14 @@ constexpr int FREC = 8;      // doubles per face record
 constexpr int GEO = 4;       // doubles per cell geometry row
 constexpr int XPW = 8;       // doubles per packed cell row
 
+// Packed cell row: [position | primitive | 0] in 3D, [primitive | position |
+// 0 0] in 2D, where the four primitives fill the row's first aligned 32-byte
+// sector, so the per-step refresh of a static mesh rewrites one whole sector.
+template <int D> struct XpLay {
+    static constexpr int X = D == 2 ? 4 : 0;     // first position component
+    static constexpr int P = D == 2 ? 0 : 3;     // first primitive
+};
+
 struct FaceArrays { double* q[5]; };
 
 // doubles per summand row: D+2 padded to even (16-byte aligned rows)

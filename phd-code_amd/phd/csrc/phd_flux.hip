@@ -10,7 +10,8 @@
 //     csr[2F]         {face id, other cell | incoming bit} per entry of each
 //                     cell's ascending face list (the order of
 //                     Tess2d/3d::extract_geometry, tess.cpp:379-382)
-//     xp[Na][8]       position(D) | primitive(D+2), packed per step  64 B/cell
+//     xp[Na][8]       position(3) | primitive(5) in 3D, primitive(4) |
+//                     position(2) | 0 0 in 2D, packed per step       64 B/cell
 //     perm[Na]        visiting order of the sliced lists (degree-sorted
 //                     windows of 256 cells)
 //     sliced lists    per slot: other cell, face id | incoming bit, face
@@ -367,9 +368,9 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_pack_xp(phd_cells c, double* xp)
     if (i >= c.n_cells) return;
     double r[XPW] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-    for (int k = 0; k < D; k++) r[k] = c.x[k][i];
+    for (int k = 0; k < D; k++) r[XpLay<D>::X + k] = c.x[k][i];
 #pragma unroll
-    for (int n = 0; n < D + 2; n++) r[D + n] = c.prim[n][i];
+    for (int n = 0; n < D + 2; n++) r[XpLay<D>::P + n] = c.prim[n][i];
     double2* o = reinterpret_cast<double2*>(xp + (size_t)i * XPW);
 #pragma unroll
     for (int q = 0; q < XPW / 2; q++) o[q] = make_double2(r[2 * q], r[2 * q + 1]);
@@ -503,9 +504,9 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
         for (int k = 0; k < D; k++) cx[k] = g[k];
         vol = g[3];
 #pragma unroll
-        for (int k = 0; k < D; k++) xi[k] = own[k];
+        for (int k = 0; k < D; k++) xi[k] = own[XpLay<D>::X + k];
 #pragma unroll
-        for (int n = 0; n < NF; n++) pv[n] = own[D + n];
+        for (int n = 0; n < NF; n++) pv[n] = own[XpLay<D>::P + n];
     }
     double pmax[NF], pmin[NF], df[NF * D];
 #pragma unroll
@@ -553,7 +554,7 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
             double cfx[D], dr[D], rr = 0.0;
 #pragma unroll
             for (int k = 0; k < D; k++) {
-                const double xj = r[k];
+                const double xj = r[XpLay<D>::X + k];
                 cfx[k] = com[k] - 0.5 * (xi[k] + xj);
                 dr[k] = xi[k] - xj;
                 rr += dr[k] * dr[k];
@@ -565,7 +566,7 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
             bool shared = div_operand_ok(rv) & (rv > 0.0);
 #pragma unroll
             for (int n = 0; n < NF; n++) {
-                const double pj = r[D + n];
+                const double pj = r[XpLay<D>::P + n];
                 pmax[n] = fmax(pmax[n], pj);
                 pmin[n] = fmin(pmin[n], pj);
                 const double d_dif = pj - pv[n];
@@ -635,7 +636,7 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
                 for (int k = 0; k < D; k++) dphi += df[n * D + k] * (cm[k] - cx[k]);
                 double psi;
                 if (dphi > 0.0 || dphi < 0.0) {
-                    double q = (xp[(size_t)j * XPW + D + n] - pv[n]) / dphi;
+                    double q = (xp[(size_t)j * XPW + XpLay<D>::P + n] - pv[n]) / dphi;
                     psi = (0. > q) ? 0. : q;
                 } else {
                     psi = 1.0;
@@ -1023,8 +1024,8 @@ __device__ __forceinline__ void store_xp_prim(double* xp, long long i, const dou
 {
     double* r = xp + (size_t)i * XPW;
     if (D == 2) {
-        reinterpret_cast<double2*>(r)[1] = make_double2(p[0], p[1]);
-        reinterpret_cast<double2*>(r)[2] = make_double2(p[2], p[3]);
+        reinterpret_cast<double2*>(r)[0] = make_double2(p[0], p[1]);
+        reinterpret_cast<double2*>(r)[1] = make_double2(p[2], p[3]);
     } else {
         r[3] = p[0];
         reinterpret_cast<double2*>(r)[2] = make_double2(p[1], p[2]);
@@ -1161,9 +1162,9 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_ghost_refresh_eos(phd_cells c, in
         if (xp) {
             double r[XPW] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-            for (int k = 0; k < D; k++) r[k] = c.x[k][i];
+            for (int k = 0; k < D; k++) r[XpLay<D>::X + k] = c.x[k][i];
 #pragma unroll
-            for (int n = 0; n < NF; n++) r[D + n] = p[n];
+            for (int n = 0; n < NF; n++) r[XpLay<D>::P + n] = p[n];
             double2* o = reinterpret_cast<double2*>(xp + (size_t)i * XPW);
 #pragma unroll
             for (int q = 0; q < XPW / 2; q++) o[q] = make_double2(r[2 * q], r[2 * q + 1]);
