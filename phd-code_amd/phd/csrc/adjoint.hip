@@ -201,11 +201,12 @@ struct CsrNbr {
     }
     PHD_HD void geo(int e, double* cm, double& a) const
     {
-        const double* fr = frec + (size_t)csr[beg + e].x * FREC;
-        cm[0] = fr[0];
-        cm[1] = fr[1];
-        cm[2] = D == 3 ? fr[2] : 0.0;
-        a = fr[3];
+        using F = FrLay<D>;
+        const double* fr = frec + (size_t)csr[beg + e].x * F::W;
+        cm[0] = fr[F::C];
+        cm[1] = fr[F::C + 1];
+        cm[2] = D == 3 ? fr[F::C + 2] : 0.0;
+        a = fr[F::A];
     }
     PHD_HD void out(int e, const double* lam) const
     {

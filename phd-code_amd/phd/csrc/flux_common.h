@@ -58,7 +58,14 @@ constexpr unsigned long long NOERR = ~0ull;
 constexpr unsigned long long INF_BITS = 0x7ff0000000000000ull;
 constexpr int INCOMING = (int)0x80000000u;
 constexpr int INVALID = 0x7fffffff;
-constexpr int FREC = 8;      // doubles per face record
+constexpr int FREC = 8;      // doubles per face record, at most
+
+// Face record: com(D) | area | normal(D) | |x_j - x_i|, 6 doubles in 2D and
+// 8 in 3D (rows stay 16-byte aligned for the paired loads)
+template <int D> struct FrLay {
+    static constexpr int W = 2 * D + 2;
+    static constexpr int C = 0, A = D, N = D + 1, DIFF = 2 * D + 1;
+};
 constexpr int GEO = 4;       // doubles per cell geometry row
 constexpr int XPW = 8;       // doubles per packed cell row
 
@@ -122,13 +129,14 @@ template <int D>
 __device__ __forceinline__ void load_face_rec(const double* frec, long long m, double* com,
                                               double& area, double* n, double& diff)
 {
-    double r[FREC];
+    using F = FrLay<D>;
+    double r[F::W];
 #pragma unroll
-    for (int q = 0; q < FREC / 2; q++) ld2(frec + (size_t)m * FREC + 2 * q, r[2 * q], r[2 * q + 1]);
+    for (int q = 0; q < F::W / 2; q++) ld2(frec + (size_t)m * F::W + 2 * q, r[2 * q], r[2 * q + 1]);
 #pragma unroll
-    for (int k = 0; k < D; k++) { com[k] = r[k]; n[k] = r[4 + k]; }
-    area = r[3];
-    diff = r[7];
+    for (int k = 0; k < D; k++) { com[k] = r[F::C + k]; n[k] = r[F::N + k]; }
+    area = r[F::A];
+    diff = r[F::DIFF];
 }
 
 template <int D>

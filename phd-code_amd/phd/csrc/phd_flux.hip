@@ -5,7 +5,8 @@
 //   API arrays (caller-owned, the reference's named fields): structure of
 //   arrays, one double array per field; reals first, then ghosts.
 //   Plan (library-owned, one per frozen mesh; phd_plan_create):
-//     face_rec[F][8]  com(3) | area | normal(3) | |x_j - x_i|      64 B/face
+//     face_rec[F][W]  com(D) | area | normal(D) | |x_j - x_i|, W = 2D + 2:
+//                     48 B/face in 2D, 64 B/face in 3D
 //     cell_geo[Na][4] centroid x+dcom (3) | volume                  32 B/cell
 //     csr[2F]         {face id, other cell | incoming bit} per entry of each
 //                     cell's ascending face list (the order of
@@ -202,21 +203,22 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_plan_faces(phd_cells c, phd_faces
 {
     long long m = xcd_index(f.n_faces);
     if (m >= f.n_faces) return;
+    using F = FrLay<D>;
     int i = f.pair_i[m], j = f.pair_j[m];
-    double r[FREC] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double r[F::W];
     double diff = 0.0;
 #pragma unroll
     for (int k = 0; k < D; k++) {
-        r[k] = f.com[k][m];
-        r[4 + k] = f.normal[k][m];
+        r[F::C + k] = f.com[k][m];
+        r[F::N + k] = f.normal[k][m];
         double dx = c.x[k][j] - c.x[k][i];
         diff += dx * dx;          // reconstruction.pyx:866
     }
-    r[3] = f.area[m];
-    r[7] = sqrt(diff);            // reconstruction.pyx:875
-    double2* o = reinterpret_cast<double2*>(rec + (size_t)m * FREC);
+    r[F::A] = f.area[m];
+    r[F::DIFF] = sqrt(diff);      // reconstruction.pyx:875
+    double2* o = reinterpret_cast<double2*>(rec + (size_t)m * F::W);
 #pragma unroll
-    for (int q = 0; q < FREC / 2; q++) o[q] = make_double2(r[2 * q], r[2 * q + 1]);
+    for (int q = 0; q < F::W / 2; q++) o[q] = make_double2(r[2 * q], r[2 * q + 1]);
 }
 
 template <int D>
