@@ -247,13 +247,15 @@ def adjoint_timing(n=256, n_steps=50, every=10, reps=3):
             "ratio": round(bwd / fwd, 3)}
 
 
-def ensemble_timing(n_problems=4096, steps=100, per_problem_steps=3, seed=0, log=None):
+def ensemble_timing(n_problems=4096, steps=100, seed=0, log=None):
     """Batched ensemble of independent 2D Riemann problems (cells per problem
-    log-uniform in [1k, 64k]): autotuned schedule, one graph launch per step,
-    timed over `steps` steps; the per-problem form (each problem through
-    phd_core_step + phd_ghost_refresh_eos on its own plan, with the batched
-    run's per-problem time steps, so no host synchronisation) timed over
-    `per_problem_steps` steps; one step of both compared bit for bit."""
+    log-uniform in [1k, 64k]): autotuned schedule, one graph launch per step.
+    Both forms run the same `steps` steps from the same initial state and are
+    timed over steps 1 .. steps-1 (step 0 warms each form): the batched run,
+    then the per-problem form (each problem through phd_core_step +
+    phd_ghost_refresh_eos on its own plan, with the batched run's per-problem
+    time steps, so no host synchronisation).  The two final states are
+    compared bit for bit."""
     import torch
     from phd import ensemble as E
     log = log or (lambda m: None)
@@ -270,52 +272,45 @@ def ensemble_timing(n_problems=4096, steps=100, per_problem_steps=3, seed=0, log
     log("ensemble device set-up %.1fs" % (time.time() - t1))
     best, tried = E.autotune(ens, steps=3, log=log)
     s0 = ens.state()
-    # timed batched run
+    # batched run
     ens.restart(s0)
     ens.run(2)                                  # warm
     ens.restart(s0)
+    ens.run(1)
     torch.cuda.synchronize()
     tb = time.perf_counter()
-    ens.run(steps)
+    ens.run(steps - 1)
     torch.cuda.synchronize()
     wall_b = time.perf_counter() - tb
     _, k, hist, faults = ens.read(history=True)
     ens.clear_errors()            # frozen problems' failed updates land there too
     assert k == steps
-    # parity of one step at full size: batched vs per problem
-    t2 = time.time()
-    ens.restart(s0)
-    ens.run(1)
-    torch.cuda.synchronize()
     after_b = ens.state()
+    # per-problem form over the same steps
+    t2 = time.time()
     ens.set_state(s0)
     views = ens.problem_views()
     log("per-problem plans %.1fs" % (time.time() - t2))
     ens.step_problems(hist[0], first=True)
     torch.cuda.synchronize()
-    same = all(torch.equal(a, b) for a, b in zip(after_b, ens.state()))
-    del after_b
-    # timed per-problem run (steps 1.. of the history)
-    ens.set_state(s0)
-    ens.step_problems(hist[0], first=True)
-    torch.cuda.synchronize()
     tp = time.perf_counter()
-    for s in range(1, 1 + per_problem_steps):
+    for s in range(1, steps):
         ens.step_problems(hist[s])
     torch.cuda.synchronize()
     wall_p = time.perf_counter() - tp
     ens.clear_errors()
+    same = all(torch.equal(a, b) for a, b in zip(after_b, ens.state()))
+    del after_b
     nf = ens.n_faces
-    ms_b = wall_b * 1e3 / steps
-    ms_p = wall_p * 1e3 / per_problem_steps
+    ms_b = wall_b * 1e3 / (steps - 1)
+    ms_p = wall_p * 1e3 / (steps - 1)
     out = {"workload": "%d independent 2D Riemann problems (static jittered-lattice Voronoi "
                        "meshes, cells log-uniform in [1k, 64k], reflective unit box, HLLC + "
                        "PieceWiseLinear(arepo, gizmo), CFL 0.3 per problem), FP64" % n_problems,
-           "cells": ens.n_cells, "faces": nf, "steps": steps,
+           "cells": ens.n_cells, "faces": nf, "steps": steps, "steps_timed": steps - 1,
            "batched_ms_per_step": round(ms_b, 3),
            "batched_face_updates_per_s": nf / (ms_b * 1e-3),
            "per_problem_ms_per_step": round(ms_p, 3),
-           "per_problem_steps_timed": per_problem_steps,
            "speedup": round(ms_p / ms_b, 2),
            "schedule": dict(zip(("grad_block", "face_block", "update_block", "xcd"), best)),
            "autotune_ms": {"%d/%d/%d/%d" % c: round(v, 3) for c, v in sorted(tried.items())},
