@@ -73,17 +73,44 @@ def algorithmic_bytes(dim, nr, ng, nf):
     return b
 
 
+def code_text(text):
+    """C/C++ source without its comments, whitespace runs collapsed (string
+    and character literals kept as written)."""
+    out, i, n = [], 0, len(text)
+    while i < n:
+        c = text[i]
+        if c in "\"'":
+            j = i + 1
+            while j < n and text[j] != c:
+                j += 2 if text[j] == "\\" else 1
+            out.append(text[i:j + 1])
+            i = j + 1
+        elif text.startswith("//", i):
+            j = text.find("\n", i)
+            i = n if j < 0 else j
+        elif text.startswith("/*", i):
+            j = text.find("*/", i + 2)
+            out.append(" ")
+            i = n if j < 0 else j + 2
+        else:
+            out.append(c)
+            i += 1
+    return " ".join("".join(out).split())
+
+
 def source_hash():
-    """sha256 over the HIP sources and the C header: a traffic record is
-    used only for the kernels it was measured on."""
+    """sha256 over the code of the HIP sources and the C header (comments
+    and layout excluded): a traffic record is used only for the kernels it
+    was measured on."""
     import hashlib
     hsh = hashlib.sha256()
     files = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
                    if f.endswith((".hip", ".h")))
     files.append(os.path.join(ROOT, "include", "phd_flux.h"))
     for fn in files:
-        with open(fn, "rb") as fh:
-            hsh.update(os.path.basename(fn).encode() + b"\0" + fh.read())
+        with open(fn, encoding="utf-8") as fh:
+            code = code_text(fh.read())
+        hsh.update(os.path.basename(fn).encode() + b"\0" + code.encode())
     return hsh.hexdigest()
 
 

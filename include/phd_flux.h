@@ -139,7 +139,8 @@ int64_t phd_plan_bytes(const phd_plan* p);
  * centroid(dim), padded to even (24 in 3D, 14 in 2D).  The gradient of cell
  * c, component g = n*dim+k, is rec[c*ld + dim + 2 + g]. */
 int phd_cell_record_ld(int dim);
-/* Refresh the plan's packed [position | primitive] rows from the cells. */
+/* Refresh the plan's packed rows (position and primitive of each cell) from
+ * the cells. */
 int phd_pack_cells(phd_handle* h, phd_plan* p, const phd_cells* c);
 
 /* ---- equation of state (equation_state/equation_state.pyx) --------------- */
@@ -271,8 +272,9 @@ typedef struct {
 typedef struct phd_batch phd_batch;
 /* cell_start: host array of n_problems + 1 offsets; hist_steps: capacity of
  * the per-step time-step history (steps x problems, 0 for none).
- * Allocating and synchronising; fails with PHD_E_ARG if a face joins two
- * problems. */
+ * Allocating and synchronising (it also reads the tags, to list the ghost
+ * cells); fails with PHD_E_ARG if a face joins two problems or a tag is
+ * neither 0 nor 1. */
 int phd_batch_create(phd_handle* h, phd_plan* p, const phd_cells* c,
                      const phd_faces* f, int32_t n_problems,
                      const int32_t* cell_start, int64_t hist_steps,
@@ -282,15 +284,20 @@ int phd_batch_destroy(phd_batch* b);
  * graph).  Default {256, 256, 256, 1}. */
 int phd_batch_set_schedule(phd_batch* b, const phd_schedule* s);
 /* Ghost refresh + EOS of phd_ghost_refresh_eos over the union, also folding
- * each problem's CFL minimum and fault flag for the next step.  Call after
- * any change of the conserved fields made outside the batch's steps. */
+ * each problem's CFL minimum and fault flag for the next step, and writing
+ * the packed rows whole (positions included).  Call after any change of the
+ * conserved fields made outside the batch's steps. */
 int phd_batch_refresh(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c,
                       int boundary, const double* lo, const double* hi, double gamma,
                       int boost);
 /* One step of every problem on h's stream, no host synchronisation:
  * dt[p] = cfl * min over p's real cells of the CFL bound (riemann.pyx:
- * 120-188) as the last refresh folded it, the core step of phd_core_step
- * with that dt per problem, the batch refresh, and the history record.
+ * 120-188) as the last step or refresh folded it, the core step of
+ * phd_core_step with that dt per problem, whose update also evaluates the
+ * EOS of the real cells, refreshes their packed rows' primitives and folds
+ * the next CFL minimum, then the ghosts' refresh (the fields equal those of
+ * phd_core_step + phd_ghost_refresh_eos bit for bit), and the history
+ * record.  The packed rows' positions are not rewritten (static meshes).
  * flags: PHD_CORE_XP_FRESH as for phd_core_step. */
 int phd_batch_step(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c,
                    const phd_faces* f, double* rec, int32_t rec_ld, int boundary,

@@ -11,6 +11,7 @@
 import ctypes
 import os
 import re
+import sys
 
 import numpy as np
 import pytest
@@ -149,3 +150,18 @@ def test_snapshot_tree_round_trip(tmp_path):
             phd.Hdf5().write("snap", str(tmp_path), Integ())
         else:
             raise RuntimeError("h5py present: the Hdf5 writer is exercised instead")
+
+
+def test_traffic_record_hash_follows_code_only():
+    """bench.py keys its measured traffic record by a hash of the kernel
+    sources' code: comments and layout do not change it, code does, and
+    string literals (which may hold comment markers) are kept."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    a = "int f(int x) { return x + 1; } // one\n/* two\n */\n"
+    b = "int f(int x)\n{\n    return x + 1;   /* one */\n}\n"
+    assert bench.code_text(a) == bench.code_text(b)
+    assert bench.code_text(a) != bench.code_text(a.replace("x + 1", "x + 2"))
+    s = 'const char* u = "a//b/*c*/"; char q = \'/\';'
+    assert bench.code_text(s) == s
