@@ -44,9 +44,9 @@ inline Gas make_gas(double gamma)
 // u), so most faces land here.  For |x - 1| (|y| + 3) <= 2^-20 the binomial
 // series (1 + d)^y = 1 + y d + y(y-1)/2 d^2 + ... is cut after the quadratic
 // term: the rest is below (|d|(|y| + 3))^3 / 6 < 2^-62, under a thousandth of
-// an ulp of 1, and the Horner sum rounds once at the final 1 + (...), so the result
-// is within an ulp of the exact power (libm's pow is within one ulp of it
-// too).  x - 1 is exact there (Sterbenz), and x == 1 gives exactly 1, as
+// an ulp of 1, and the Horner sum rounds once at the final 1 + (...), so the
+// result is within an ulp of the exact power (libm's pow is within one ulp of
+// it too).  x - 1 is exact there (Sterbenz), and x == 1 gives exactly 1, as
 // pow(1, y) does (C99 F.9.4.4).  A wave whose lanes all take the series never
 // executes the pow sequence.
 PHD_HD __forceinline__ double pow_b1(double x, double y)
@@ -127,47 +127,46 @@ PHD_HD __forceinline__ void riemann_hll(const double* L, const double* R, const 
     if (boost) wn = 0.;
     double sl, sc, sr;
     get_waves(_dl, Vnl, _pl, _dr, Vnr, _pr, g, sl, sc, sr);
-    if (wn <= sl) {
-        side_flux<D>(L, n, Vnl, vl_sq, wn, gamma, F);
-    } else if ((sl < wn) && (wn <= sr)) {
-        if (SOLVER == PHD_SOLVER_HLL) {
-            // Toro eq. 10.20-10.21 (riemann.pyx:346-361)
-            double fac1 = sr - wn, fac2 = sl - wn, fac3 = sr - sl;
-            F[0] = (_dl * Vnl * fac1 - _dr * Vnr * fac2 - sl * _dl * fac1 + sr * _dr * fac2) / fac3;
+    if (SOLVER == PHD_SOLVER_HLL && (sl < wn) && (wn <= sr)) {
+        // Toro eq. 10.20-10.21 (riemann.pyx:346-361)
+        double fac1 = sr - wn, fac2 = sl - wn, fac3 = sr - sl;
+        F[0] = (_dl * Vnl * fac1 - _dr * Vnr * fac2 - sl * _dl * fac1 + sr * _dr * fac2) / fac3;
 #pragma unroll
-            for (int k = 0; k < D; k++)
-                F[1 + k] = ((_dl * L[1 + k] * Vnl + _pl * n[k]) * fac1 -
-                            (_dr * R[1 + k] * Vnr + _pr * n[k]) * fac2 -
-                            sl * (_dl * L[1 + k]) * fac1 + sr * (_dr * R[1 + k]) * fac2) / fac3;
-            double el = 0.5 * _dl * vl_sq + _pl / (gamma - 1.0);
-            double er = 0.5 * _dr * vr_sq + _pr / (gamma - 1.0);
-            F[D + 1] = ((el + _pl) * Vnl * fac1 - (er + _pr) * Vnr * fac2 - sl * el * fac1 +
-                        sr * er * fac2) / fac3;
-        } else if (wn <= sc) {
-            // left star state, Toro eq. 10.38-10.39 (riemann.pyx:636-652)
-            double factor_1 = _dl * (sl - Vnl) / (sl - sc);
-            double factor_2 = factor_1 * (sl - wn) * (sc - Vnl) + _pl;
-            double frho = _dl * (Vnl - sl) + factor_1 * (sl - wn);
-            double el = 0.5 * _dl * vl_sq + _pl / (gamma - 1.0);
-            F[0] = frho;
-            F[D + 1] = (el + _pl) * Vnl - sl * el +
-                       (sl - wn) * factor_1 * (el / _dl + (sc - Vnl) * (sc + _pl / (_dl * (sl - Vnl))));
-#pragma unroll
-            for (int k = 0; k < D; k++) F[1 + k] = frho * L[1 + k] + factor_2 * n[k];
-        } else {
-            // right star state (riemann.pyx:654-670)
-            double factor_1 = _dr * (sr - Vnr) / (sr - sc);
-            double factor_2 = factor_1 * (sr - wn) * (sc - Vnr) + _pr;
-            double frho = _dr * (Vnr - sr) + factor_1 * (sr - wn);
-            double er = 0.5 * _dr * vr_sq + _pr / (gamma - 1.0);
-            F[0] = frho;
-            F[D + 1] = (er + _pr) * Vnr - sr * er +
-                       (sr - wn) * factor_1 * (er / _dr + (sc - Vnr) * (sc + _pr / (_dr * (sr - Vnr))));
-#pragma unroll
-            for (int k = 0; k < D; k++) F[1 + k] = frho * R[1 + k] + factor_2 * n[k];
-        }
+        for (int k = 0; k < D; k++)
+            F[1 + k] = ((_dl * L[1 + k] * Vnl + _pl * n[k]) * fac1 -
+                        (_dr * R[1 + k] * Vnr + _pr * n[k]) * fac2 -
+                        sl * (_dl * L[1 + k]) * fac1 + sr * (_dr * R[1 + k]) * fac2) / fac3;
+        double el = 0.5 * _dl * vl_sq + _pl / (gamma - 1.0);
+        double er = 0.5 * _dr * vr_sq + _pr / (gamma - 1.0);
+        F[D + 1] = ((el + _pl) * Vnl * fac1 - (er + _pr) * Vnr * fac2 - sl * el * fac1 +
+                    sr * er * fac2) / fac3;
     } else {
-        side_flux<D>(R, n, Vnr, vr_sq, wn, gamma, F);
+        // The two upwind fluxes (riemann.pyx:624-631, 672-679) and the two
+        // HLLC star fluxes (Toro eq. 10.38-10.39, 636-652 and 654-670) are
+        // each one expression applied to the left or the right side's data:
+        // each lane selects its side and the expression is evaluated once,
+        // so a wave with faces on both sides runs one evaluation, and every
+        // lane computes the operations of its own branch.
+        const bool mid = !(wn <= sl) && (sl < wn) && (wn <= sr);
+        const bool left = mid ? (wn <= sc) : (wn <= sl);
+        double S[D + 2];
+#pragma unroll
+        for (int q = 0; q < D + 2; q++) S[q] = left ? L[q] : R[q];
+        const double d = S[0], p = S[D + 1];
+        const double Vn = left ? Vnl : Vnr, vsq = left ? vl_sq : vr_sq, s = left ? sl : sr;
+        if (SOLVER == PHD_SOLVER_HLLC && mid) {
+            double factor_1 = d * (s - Vn) / (s - sc);
+            double factor_2 = factor_1 * (s - wn) * (sc - Vn) + p;
+            double frho = d * (Vn - s) + factor_1 * (s - wn);
+            double e = 0.5 * d * vsq + p / (gamma - 1.0);
+            F[0] = frho;
+            F[D + 1] = (e + p) * Vn - s * e +
+                       (s - wn) * factor_1 * (e / d + (sc - Vn) * (sc + p / (d * (s - Vn))));
+#pragma unroll
+            for (int k = 0; k < D; k++) F[1 + k] = frho * S[1 + k] + factor_2 * n[k];
+        } else {
+            side_flux<D>(S, n, Vn, vsq, wn, gamma, F);
+        }
     }
 }
 
