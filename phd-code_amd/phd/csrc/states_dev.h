@@ -46,11 +46,18 @@ PHD_HD __forceinline__ void gizmo_limit(double sepi_mag, double sepj_mag, double
         double phi_max = fmax(sl[n], sr[n]);
         double phibar_l = sl[n] + sepi_mag / diff * (sr[n] - sl[n]);
         double phibar_r = sr[n] + sepj_mag / diff * (sl[n] - sr[n]);
-        double phi_plus, phi_minus;
-        if ((phi_max + delta1) * phi_max >= 0.) phi_plus = phi_max + delta1;
-        else phi_plus = phi_max / (1 + delta1 / fabs(phi_max));
-        if ((phi_min - delta1) * phi_min >= 0.) phi_minus = phi_min - delta1;
-        else phi_minus = phi_min / (1 + delta1 / fabs(phi_min));
+        // the two fallbacks (892-900) exclude each other for finite values:
+        // the first needs phi_max < 0, the second phi_min > 0, and phi_min <=
+        // phi_max; so one evaluation serves the lane's case
+        const bool fp = !((phi_max + delta1) * phi_max >= 0.);
+        const bool fm = !((phi_min - delta1) * phi_min >= 0.);
+        double phi_plus = phi_max + delta1, phi_minus = phi_min - delta1;
+        if (fp || fm) {
+            const double x = fp ? phi_max : phi_min;
+            const double y = x / (1 + delta1 / fabs(x));
+            if (fp) phi_plus = y;
+            else phi_minus = y;
+        }
         if (pi[n] < pj[n]) {
             L[n] = fmax(phi_minus, fmin(phibar_l + delta2, L[n]));
             R[n] = fmin(phi_plus, fmax(phibar_r - delta2, R[n]));
