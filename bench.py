@@ -274,7 +274,7 @@ def adjoint_timing(n=256, n_steps=50, every=10, reps=3):
             "ratio": round(bwd / fwd, 3)}
 
 
-def ensemble_timing(n_problems=4096, steps=100, seed=0, log=None):
+def ensemble_timing(n_problems=4096, steps=100, seed=0, log=None, schedule=None):
     """Batched ensemble of independent 2D Riemann problems (cells per problem
     log-uniform in [1k, 64k]): autotuned schedule, one graph launch per step.
     Both forms run the same `steps` steps from the same initial state and are
@@ -282,7 +282,8 @@ def ensemble_timing(n_problems=4096, steps=100, seed=0, log=None):
     then the per-problem form (each problem through phd_core_step +
     phd_ghost_refresh_eos on its own plan, with the batched run's per-problem
     time steps, so no host synchronisation).  The two final states are
-    compared bit for bit."""
+    compared bit for bit.  schedule: a fixed (grad, face, update block, xcd)
+    schedule instead of the autotuner's search."""
     import torch
     from phd import ensemble as E
     log = log or (lambda m: None)
@@ -297,7 +298,8 @@ def ensemble_timing(n_problems=4096, steps=100, seed=0, log=None):
     ens.reset()
     torch.cuda.synchronize()
     log("ensemble device set-up %.1fs" % (time.time() - t1))
-    best, tried = E.autotune(ens, steps=3, log=log)
+    best, tried = E.autotune(ens, steps=3, log=log,
+                             candidates=[tuple(schedule)] if schedule else None)
     s0 = ens.state()
     # batched run
     ens.restart(s0)
@@ -397,6 +399,9 @@ def main():
     ap.add_argument("--workload", choices=("sedov", "ensemble"), default="sedov",
                     help="ensemble: only the batched-ensemble measurement, as its own line")
     ap.add_argument("--problems", type=int, default=4096)
+    ap.add_argument("--schedule", default=None, metavar="G,F,U,X",
+                    help="ensemble: a fixed schedule (threads per block of the gradient, "
+                         "face and update kernels, XCD remap 0/1) instead of the autotuner")
     ap.add_argument("--sq-from", default=None, metavar="SQ_CSV",
                     help="with --traffic-from: a third rocprofv3 --pmc run with SQ_INSTS_VALU")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -424,7 +429,8 @@ def main():
     if args.workload == "ensemble":
         def log(m):
             print(m, file=sys.stderr, flush=True)
-        r = ensemble_timing(args.problems, steps=args.steps, log=log)
+        sched = [int(v) for v in args.schedule.split(",")] if args.schedule else None
+        r = ensemble_timing(args.problems, steps=args.steps, log=log, schedule=sched)
         r.update({"metric": "batched ensemble face-updates/s FP64 (one graph launch per step) "
                             "and speed-up over per-problem launches",
                   "value": r["batched_face_updates_per_s"], "unit": "face-updates/s",
