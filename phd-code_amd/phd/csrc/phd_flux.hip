@@ -980,14 +980,35 @@ __global__ __launch_bounds__(BS) void k_update_sell(phd_cells c,
                                                            const int2* __restrict__ cinfo,
                                                            unsigned long long* err)
 {
+    // At SORT_WINDOW threads a block owns one whole window (the degree sort
+    // permutes within windows): the updated values pass through LDS and are
+    // stored in natural cell order, whole cache lines per wave.
     constexpr int NF = D + 2;
-    const long long t = sched_index<BS, XCD>(c.n_cells);
-    if (t >= c.n_cells) return;
-    const int i = perm[t];
+    constexpr bool STAGE = BS == SORT_WINDOW;
+    __shared__ double su[STAGE ? NF * SORT_WINDOW : 1];
+    const long long n = c.n_cells;
+    const long long t = sched_index<BS, XCD>(n);
+    const long long w0 = t - threadIdx.x;          // the window's first cell
+    const int i = t < n ? perm[t] : 0;
     double u[NF];
-    if (!update_cell<D>(c, t, i, g, owner_runs, csr, off, sbase, sfi, cinfo, err, u)) return;
+    const bool real = t < n && update_cell<D>(c, t, i, g, owner_runs, csr, off, sbase, sfi, cinfo, err, u);
+    if (!STAGE) {
+        if (real) {
 #pragma unroll
-    for (int q = 0; q < NF; q++) c.cons[q][i] = u[q];
+            for (int q = 0; q < NF; q++) c.cons[q][i] = u[q];
+        }
+        return;
+    }
+    if (real) {
+#pragma unroll
+        for (int q = 0; q < NF; q++) su[q * SORT_WINDOW + (i - w0)] = u[q];
+    }
+    __syncthreads();
+    const long long k = w0 + threadIdx.x;
+    if (k < n && c.tag[k] == 0) {
+#pragma unroll
+        for (int q = 0; q < NF; q++) c.cons[q][k] = su[q * SORT_WINDOW + threadIdx.x];
+    }
 }
 
 // Static-mesh epilogue: DomainManager.update_ghost_fields(particles,
@@ -2034,22 +2055,6 @@ int batch_face(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c, con
     return PHD_E_ARG;
 }
 
-template <int D>
-int batch_update(phd_handle* h, const phd_schedule& s, phd_plan* p, const phd_cells* c)
-{
-    const long long n = c->n_cells;
-#define X(BS, XC)                                                                              \
-    if (s.update_block == BS && s.xcd == XC) {                                                 \
-        k_update_sell<D, BS, XC><<<sched_grid<BS, XC>(n), BS, 0, h->stream>>>(                 \
-            *c, p->g, p->owner_runs, p->csr, p->off, p->perm, p->sbase, p->sfi, p->cinfo,      \
-            h->d_err);                                                                         \
-        LAUNCHCHK();                                                                           \
-        return 0;                                                                              \
-    }
-    PHD_SCHED_CASES(X)
-#undef X
-    return PHD_E_ARG;
-}
 // the update fused with the real cells' epilogue, then the ghosts
 template <int D>
 int batch_update_eos(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c, int boundary,
