@@ -918,7 +918,8 @@ __device__ __forceinline__ bool update_cell(const phd_cells& c, long long t, int
                                             const int2* __restrict__ cinfo,
                                             unsigned long long* err, double* u)
 {
-    constexpr int NF = D + 2, GW = summand_width<D>(), B = 4;
+    // B summand rows in flight per batch: 3D cells have ~16 faces, 2D ~6
+    constexpr int NF = D + 2, GW = summand_width<D>(), B = D == 3 ? 8 : 4;
     const int beg = off[i], end = off[i + 1];
     if (ghost_fault<D>(c, i, beg, end, csr, err)) return false;
 #pragma unroll
