@@ -329,3 +329,72 @@ def test_pow_near_one(lib):
     assert np.all(out[x == 1.0] == 1.0)
     inside = np.abs(x - 1.0) * (np.abs(y) + 3.0) <= 2.0 ** -20
     assert inside.sum() > 0.8 * x.size and (~inside).sum() >= 100
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_spatial_states_match_oracle_bitwise(lib, dim):
+    """The host build of spatial_states (states_dev.h, the device code)
+    against the oracle's add_spatial (reconstruction.pyx:741-921) on random
+    faces, bit for bit, with inputs that take both GIZMO fallback divisions
+    (892-900): negative velocities small against their jump (phi_plus) and
+    small positive densities/pressures against a large jump (phi_minus), and
+    equal states (the third branch)."""
+    import sys
+    sys.path.insert(0, ROOT)
+    from oracle import oracle
+    rng = np.random.default_rng(5 + dim)
+    nf, nfld = 600, dim + 2
+    n = 2 * nf
+    prim = [rng.uniform(0.05, 3.0, n) for _ in range(nfld)]
+    for k in range(dim):
+        prim[1 + k] = rng.uniform(-1.0, 1.0, n)
+    # one third of the faces with a large density/pressure contrast, one third
+    # with same-signed negative velocities of very different size
+    big = rng.random(nf) < 1 / 3
+    neg = (~big) & (rng.random(nf) < 0.5)
+    for f in (0, dim + 1):
+        prim[f][0::2][big] = rng.uniform(0.01, 0.1, big.sum())
+        prim[f][1::2][big] = rng.uniform(2.0, 5.0, big.sum())
+    prim[1][0::2][neg] = rng.uniform(-0.1, -0.01, neg.sum())
+    prim[1][1::2][neg] = rng.uniform(-2.0, -1.0, neg.sum())
+    same = rng.random(nf) < 0.05
+    for f in range(nfld):
+        prim[f][1::2][same] = prim[f][0::2][same]
+    x = [rng.uniform(0, 1, n) for _ in range(dim)]
+    dcx = [rng.uniform(-0.01, 0.01, n) for _ in range(dim)]
+    grad = [rng.normal(0, 2.0, n) for _ in range(dim * nfld)]
+    com = [0.5 * (x[k][0::2] + x[k][1::2]) + rng.uniform(-0.02, 0.02, nf) for k in range(dim)]
+    fv = [rng.uniform(-0.2, 0.2, nf) for _ in range(dim)]
+    a = {"dim": dim, "nf": nf, "prim": prim, "x": x, "dcom": dcx, "com": com, "fv": fv,
+         "pair_i": np.arange(0, n, 2, dtype=np.int32), "pair_j": np.arange(1, n, 2, dtype=np.int32)}
+    for boost in (0, 1):
+        pl, pr = oracle.add_spatial(a, grad, boost=bool(boost), gizmo=True)
+        taken = {"plus": 0, "minus": 0}
+        for m in range(nf):
+            i, j = 2 * m, 2 * m + 1
+            ci = np.array([x[k][i] + dcx[k][i] for k in range(dim)])
+            cj = np.array([x[k][j] + dcx[k][j] for k in range(dim)])
+            d2 = 0.0
+            for k in range(dim):
+                d2 += (x[k][j] - x[k][i]) * (x[k][j] - x[k][i])
+            diff = np.sqrt(d2)
+            pi = np.array([prim[f][i] for f in range(nfld)])
+            pj = np.array([prim[f][j] for f in range(nfld)])
+            gi = np.array([grad[q][i] for q in range(dim * nfld)])
+            gj = np.array([grad[q][j] for q in range(dim * nfld)])
+            cm = np.array([com[k][m] for k in range(dim)])
+            fvm = np.array([fv[k][m] for k in range(dim)])
+            L, R = np.zeros(nfld), np.zeros(nfld)
+            lib.ah_spatial(dim, _p(cm), ctypes.c_double(diff), _p(ci), _p(cj), _p(pi), _p(pj),
+                           _p(gi), _p(gj), _p(fvm), boost, 1, None, None, _p(L), _p(R),
+                           None, None, None, None)
+            assert np.array_equal(L, [pl[f][m] for f in range(nfld)]), (boost, m)
+            assert np.array_equal(R, [pr[f][m] for f in range(nfld)]), (boost, m)
+            sl = pi - (np.r_[0.0, fvm, 0.0] if boost else 0.0)
+            sr = pj - (np.r_[0.0, fvm, 0.0] if boost else 0.0)
+            for f in range(nfld):
+                d1 = 0.5 * abs(sl[f] - sr[f])
+                hi, lo = max(sl[f], sr[f]), min(sl[f], sr[f])
+                taken["plus"] += (hi + d1) * hi < 0
+                taken["minus"] += (lo - d1) * lo < 0
+        assert taken["plus"] > 20 and taken["minus"] > 20, taken
