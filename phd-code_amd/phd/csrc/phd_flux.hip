@@ -1314,10 +1314,13 @@ int check_plan_faces(const phd_plan* p, const phd_faces* f)
     return 0;
 }
 
-int check_rec(int dim, const double* rec, int ld)
+// cell-record rows: ld >= the layout's width, even, 16-byte aligned; the
+// array may be null only when there are no cells
+int check_rec(const phd_cells* c, const double* rec, int ld)
 {
-    int need = dim == 3 ? Lay<3>::LD : Lay<2>::LD;
-    if (!rec || ld < need || (ld & 1) || (((uintptr_t)rec) & 15)) return PHD_E_ARG;
+    int need = c->dim == 3 ? Lay<3>::LD : Lay<2>::LD;
+    if ((!rec && c->n_cells > 0) || ld < need || (ld & 1) || (((uintptr_t)rec) & 15))
+        return PHD_E_ARG;
     return 0;
 }
 
@@ -1733,7 +1736,7 @@ int phd_pack_cells(phd_handle* h, phd_plan* p, const phd_cells* c)
 int phd_compute_gradients(phd_handle* h, phd_plan* p, const phd_cells* c, int limiter,
                           double* rec, int32_t rec_ld)
 {
-    if (!h || check_cells(c) || check_plan(p, c) || check_rec(c->dim, rec, rec_ld)) return PHD_E_ARG;
+    if (!h || check_cells(c) || check_plan(p, c) || check_rec(c, rec, rec_ld)) return PHD_E_ARG;
     int rc = phd_pack_cells(h, p, c);
     if (rc) return rc;
     return c->dim == 2 ? launch_grad<2>(h, p, c, limiter, rec, rec_ld)
@@ -1743,7 +1746,7 @@ int phd_compute_gradients(phd_handle* h, phd_plan* p, const phd_cells* c, int li
 int phd_update_ghost_gradients(phd_handle* h, phd_plan* p, const phd_cells* c, int boundary,
                                const double* lo, const double* hi, double* rec, int32_t rec_ld)
 {
-    if (!h || check_cells(c) || check_plan(p, c) || check_rec(c->dim, rec, rec_ld)) return PHD_E_ARG;
+    if (!h || check_cells(c) || check_plan(p, c) || check_rec(c, rec, rec_ld)) return PHD_E_ARG;
     return c->dim == 2 ? launch_ghost<2>(h, p, c, boundary, lo, hi, rec, rec_ld)
                        : launch_ghost<3>(h, p, c, boundary, lo, hi, rec, rec_ld);
 }
@@ -1756,7 +1759,7 @@ int phd_compute_states(phd_handle* h, phd_plan* p, const phd_cells* c, const phd
     if (!h || check_cells(c) || check_faces(f) || check_plan(p, c) || check_plan_faces(p, f) ||
         !left || !right)
         return PHD_E_ARG;
-    if (order == PHD_ORDER_LINEAR && check_rec(c->dim, rec, rec_ld)) return PHD_E_ARG;
+    if (order == PHD_ORDER_LINEAR && check_rec(c, rec, rec_ld)) return PHD_E_ARG;
     if (order != PHD_ORDER_LINEAR && order != PHD_ORDER_CONSTANT) return PHD_E_MODE;
     if (f->n_faces == 0) return 0;
     const int nf = c->dim + 2;
@@ -1807,7 +1810,7 @@ int phd_face_flux(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_face
     if (!h || check_cells(c) || check_faces(f) || check_plan(p, c) || check_plan_faces(p, f) ||
         !flux)
         return PHD_E_ARG;
-    if (order == PHD_ORDER_LINEAR && check_rec(c->dim, rec, rec_ld)) return PHD_E_ARG;
+    if (order == PHD_ORDER_LINEAR && check_rec(c, rec, rec_ld)) return PHD_E_ARG;
     FaceArrays F = fa(flux, c->dim + 2);
     return c->dim == 2
                ? launch_face<2, 1>(h, p, c, f, rec, rec_ld, gamma, dt, 0.0, boost, gizmo, order, solver, F)
@@ -1830,7 +1833,7 @@ int phd_update_from_fluxes(phd_handle* h, phd_plan* p, const phd_cells* c, const
 int phd_gradients_packed(phd_handle* h, phd_plan* p, const phd_cells* c, int limiter,
                          double* rec, int32_t rec_ld)
 {
-    if (!h || check_cells(c) || check_plan(p, c) || check_rec(c->dim, rec, rec_ld)) return PHD_E_ARG;
+    if (!h || check_cells(c) || check_plan(p, c) || check_rec(c, rec, rec_ld)) return PHD_E_ARG;
     return c->dim == 2 ? launch_grad<2>(h, p, c, limiter, rec, rec_ld)
                        : launch_grad<3>(h, p, c, limiter, rec, rec_ld);
 }
@@ -1840,7 +1843,7 @@ int phd_face_summands(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_
                       int gizmo, int solver, double* const* flux)
 {
     if (!h || check_cells(c) || check_faces(f) || check_plan(p, c) || check_plan_faces(p, f) ||
-        check_rec(c->dim, rec, rec_ld))
+        check_rec(c, rec, rec_ld))
         return PHD_E_ARG;
     const int D = c->dim;
     FaceArrays F = fa(flux, D + 2);
@@ -1866,7 +1869,7 @@ int phd_core_step(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_face
                   double* const* flux, int flags)
 {
     if (!h || check_cells(c) || check_faces(f) || check_plan(p, c) || check_plan_faces(p, f) ||
-        check_rec(c->dim, rec, rec_ld))
+        check_rec(c, rec, rec_ld))
         return PHD_E_ARG;
     int rc = 0;
     if (!(flags & PHD_CORE_XP_FRESH)) rc = phd_pack_cells(h, p, c);
@@ -2142,7 +2145,7 @@ int batch_args_ok(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c, 
                   const double* rec, int ld, int boundary, const double* lo, const double* hi)
 {
     if (!h || !b || check_cells(c) || check_faces(f) || check_plan(p, c) ||
-        check_plan_faces(p, f) || check_rec(c->dim, rec, ld))
+        check_plan_faces(p, f) || check_rec(c, rec, ld))
         return PHD_E_ARG;
     if (b->dim != c->dim || b->n_cells != c->n_cells || b->n_faces != f->n_faces) return PHD_E_ARG;
     if (boundary != PHD_BOUNDARY_REFLECTIVE && boundary != PHD_BOUNDARY_PERIODIC) return PHD_E_MODE;
