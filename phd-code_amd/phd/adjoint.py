@@ -206,7 +206,11 @@ class Lane(object):
         nat.check(st.lib.phd_handle_create(ctypes.byref(hp), dev.index), "phd_handle_create")
         self.h = hp.value
         st.lib.phd_set_stream(self.h, ctypes.c_void_p(self.stream.cuda_stream))
-        self.cons = [torch.empty_like(pc[f]) for f in st.cons_names]
+        # the conserved fields as rows of one buffer: a snapshot is one copy
+        ref = pc[st.cons_names[0]]
+        self.cons_buf = torch.empty((len(st.cons_names), ref.numel()), dtype=ref.dtype,
+                                    device=dev)
+        self.cons = list(self.cons_buf.unbind(0))
         self.prim = [torch.empty_like(pc[f]) for f in st.prim_names]
         self.cells = st.view(self.cons, self.prim)
         self.plan = nat.Plan(self.cells, st.faces)
@@ -289,8 +293,10 @@ class Rollout(torch.autograd.Function):
         prim_tmp = [torch.empty_like(pc[f]) for f in st.prim_names]
         sets = []
         for _ in range(2):
-            snaps = [[torch.empty_like(pc[f]) for f in st.cons_names] for _ in range(nseg)]
-            sets.append({"snaps": snaps, "views": [st.view(sn, prim_tmp) for sn in snaps],
+            snap_bufs = [torch.empty_like(lane.cons_buf) for _ in range(nseg)]
+            snaps = [list(sb.unbind(0)) for sb in snap_bufs]
+            sets.append({"snap_bufs": snap_bufs,
+                         "views": [st.view(sn, prim_tmp) for sn in snaps],
                          "recs": [st.new_rec() for _ in range(nseg)],
                          "ready": torch.cuda.Event(), "free": torch.cuda.Event()})
         for b in sets:
@@ -304,8 +310,7 @@ class Rollout(torch.autograd.Function):
                 lane.stream.wait_event(b["free"])
                 lane.load(ctx.checkpoints[s0])
                 for n in range(s0, s1):
-                    for t, u in zip(b["snaps"][n - s0], lane.cons):
-                        t.copy_(u)
+                    b["snap_bufs"][n - s0].copy_(lane.cons_buf)
                     if n < s1 - 1:
                         lane.step(ctx.dts[n], b["recs"][n - s0], packed=n > s0)
                 b["ready"].record(lane.stream)
