@@ -56,8 +56,8 @@ def algorithmic_bytes(dim, nr, ng, nf):
     # gradients: cell prim/x/dcom/volume/tag, face area/com/pairs, CSR, grad out
     b["gradients"] = (na * (8 * (nfld + 2 * dim + 1) + 1) + nf * (8 * (1 + dim) + 8)
                       + 4 * (na + 1) + 4 * 2 * nf + nr * 8 * ngrad)
-    # ghost gradients: read image rows, write ghost rows, map/tag/type/x
-    b["ghost_gradients"] = ng * (2 * 8 * ngrad + 4 + 2 + 8 * dim) + na
+    # ghost gradients: read image rows, write ghost rows, ghost list/map/type/x
+    b["ghost_gradients"] = ng * (2 * 8 * ngrad + 4 + 4 + 1 + 8 * dim)
     # fused face kernel: face geometry + pairs, cell prim/x/dcom/grad, flux out
     b["face_flux"] = (nf * (8 * (3 * dim) + 8 + 8 * nfld)
                       + na * 8 * (nfld + 2 * dim + ngrad))

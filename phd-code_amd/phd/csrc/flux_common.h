@@ -48,6 +48,8 @@ struct phd_plan {
     double* g;                   // per-face update summand rows (dt*area)*flux, F x GW
     int owner_runs;              // every real cell's owned faces are one id run that
                                  // follows all its incoming faces (faces in owner order)
+    int32_t* ghosts;             // third allocation: the ghost cells (tag 1), ascending
+    int n_ghosts;
 };
 
 namespace {

@@ -667,17 +667,19 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
 // 829-834) with the reflective diagonal flip (boundary.pyx:291-307);
 // primitive and centroid are the ghost's own
 template <int D>
-__global__ __launch_bounds__(PHD_BLOCK) void k_ghost_rec(phd_cells c, const double* __restrict__ geo,
+__global__ __launch_bounds__(PHD_BLOCK) void k_ghost_rec(phd_cells c,
+                                                         const int32_t* __restrict__ gl,
+                                                         int n_ghosts,
+                                                         const double* __restrict__ geo,
                                                          int reflect, double lo0, double lo1,
                                                          double lo2, double hi0, double hi1,
                                                          double hi2, double* __restrict__ rec,
                                                          int ld)
 {
     using Y = Lay<D>;
-    long long ii = xcd_index(c.n_cells);
-    if (ii >= c.n_cells) return;
-    const int i = (int)ii;
-    if (c.tag[i] != 1) return;
+    const long long k = xcd_index(n_ghosts);     // one thread per listed ghost
+    if (k >= n_ghosts) return;
+    const int i = gl[k];
     const double* src = rec + (size_t)c.map[i] * ld;
     double out[Y::LD];
 #pragma unroll
@@ -1374,12 +1376,12 @@ int launch_ghost(phd_handle* h, phd_plan* p, const phd_cells* c, int boundary, c
     if (c->n_cells == 0) return 0;
     if (boundary != PHD_BOUNDARY_REFLECTIVE && boundary != PHD_BOUNDARY_PERIODIC) return PHD_E_MODE;
     if (boundary == PHD_BOUNDARY_REFLECTIVE && (!lo || !hi)) return PHD_E_ARG;
-    if (!c->map) return 0;
+    if (!c->map || p->n_ghosts == 0) return 0;
     double l[3], u[3];
     bounds3(D, boundary, lo, hi, l, u);
-    k_ghost_rec<D><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(
-        *c, p->cell_geo, boundary == PHD_BOUNDARY_REFLECTIVE, l[0], l[1], l[2], u[0], u[1], u[2],
-        rec, ld);
+    k_ghost_rec<D><<<xcd_grid(p->n_ghosts), PHD_BLOCK, 0, h->stream>>>(
+        *c, p->ghosts, p->n_ghosts, p->cell_geo, boundary == PHD_BOUNDARY_REFLECTIVE, l[0], l[1],
+        l[2], u[0], u[1], u[2], rec, ld);
     LAUNCHCHK();
     return 0;
 }
@@ -1432,6 +1434,27 @@ int launch_update(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_face
 
 // sliced face lists of a plan: widths on the device, bases by a host prefix
 // sum (plan creation is the synchronising, allocating call), then the slots
+// the ghost cells (tag 1) in ascending order, for the kernels that touch
+// ghosts only
+int build_ghost_list(phd_handle* h, phd_plan* p, const phd_cells* c)
+{
+    const long long na = c->n_cells;
+    if (na == 0) return 0;
+    std::vector<int8_t> tag((size_t)na);
+    HIPCHK(hipMemcpyAsync(tag.data(), c->tag, (size_t)na, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<int32_t> gl;
+    for (long long i = 0; i < na; i++)
+        if (tag[i] == 1) gl.push_back((int32_t)i);
+    p->n_ghosts = (int)gl.size();
+    if (gl.empty()) return 0;
+    HIPCHK(hipMalloc(&p->ghosts, 4 * gl.size()));
+    p->bytes += 4 * gl.size();
+    HIPCHK(hipMemcpyAsync(p->ghosts, gl.data(), 4 * gl.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
 int build_slots(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces* f)
 {
     const long long na = c->n_cells, nf = f->n_faces;
@@ -1623,6 +1646,8 @@ int phd_plan_create(phd_handle* h, const phd_cells* c, const phd_faces* f, phd_p
     p->perm = (int32_t*)b; b += s_perm;
     p->d_bad = (unsigned long long*)b;
     p->slot_base = nullptr;
+    p->ghosts = nullptr;
+    p->n_ghosts = 0;
     int rc = 0;
     do {
         if (hipMemcpyAsync(p->off, f->nbr_off, sizeof(int32_t) * (size_t)(na + 1), hipMemcpyDeviceToDevice, h->stream) != hipSuccess) { rc = PHD_E_HIP; break; }
@@ -1644,8 +1669,10 @@ int phd_plan_create(phd_handle* h, const phd_cells* c, const phd_faces* f, phd_p
         p->bad_cell = bad[0] == NOERR ? -1 : (long long)bad[0];
         p->owner_runs = bad[1] == NOERR;
         rc = build_slots(h, p, c, f);
+        if (!rc) rc = build_ghost_list(h, p, c);
     } while (0);
     if (rc) {
+        if (p->ghosts) (void)hipFree(p->ghosts);
         if (p->slot_base) (void)hipFree(p->slot_base);
         (void)hipFree(p->base);
         delete p;
@@ -1659,6 +1686,7 @@ int phd_plan_destroy(phd_plan* p)
 {
     if (!p) return PHD_E_ARG;
     (void)hipDeviceSynchronize();
+    if (p->ghosts) (void)hipFree(p->ghosts);
     if (p->slot_base) (void)hipFree(p->slot_base);
     (void)hipFree(p->base);
     delete p;
