@@ -19,6 +19,8 @@ roofline line for the dominant kernel.  The CPU baseline is the oracle
 full core step of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128]
+       python bench.py --workload ensemble [--problems 4096] [--steps 100]
+                       [--schedule G,F,U,X]    (batched ensemble, its own line)
 """
 import argparse
 import json
