@@ -127,3 +127,38 @@ def test_a_failed_problem_is_frozen_and_the_others_go_on(gpu):
     ens.clear_errors()
     for q, (a, b) in enumerate(zip(batched, ens.state())):
         assert torch.equal(a[~mask], b[~mask]), q
+
+
+def _steps_match(ens, n_steps):
+    ens.reset_history()
+    for _ in range(n_steps):
+        ens.step_eager()
+    _, k, hist, faults = ens.read(history=True)
+    assert k == n_steps and not faults.any()
+    batched = ens.state()
+    ens.check()
+    ens.reset()
+    for s in range(n_steps):
+        ref_dt = np.array([ens.time_step_problem(p) for p in range(ens.n_problems)])
+        assert np.array_equal(ref_dt, hist[s]), s
+        ens.step_problems(hist[s], first=(s == 0))
+    ens.check()
+    for q, (a, b) in enumerate(zip(batched, ens.state())):
+        assert torch.equal(a, b), (q, float((a - b).abs().max()))
+
+
+@pytest.mark.parametrize("sides", [[2, 3, 150, 5, 40, 2, 60], [70]])
+def test_ragged_and_single_problem_batches(gpu, sides):
+    """Problems of 4 to 22 500 cells in one union (several tiny problems share
+    a sort window and a wave, so the CFL fold reduces over several problems per
+    wave and the fused update stages windows that straddle problems), and a
+    batch of one problem: three batched steps equal the per-problem steps bit
+    for bit, time steps included."""
+    from phd import ensemble as E
+    print("ensemble sides=%s" % sides, flush=True)
+    spec = E.draw_problems(len(sides), seed=23)
+    spec["side"] = np.array(sides, dtype=np.int64)
+    ens = E.Ensemble(E.HostEnsemble(spec, n_workers=4), hist_steps=8)
+    ens.reset()
+    _steps_match(ens, 3)
+    ens.close()
