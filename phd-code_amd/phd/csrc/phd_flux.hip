@@ -522,8 +522,8 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
     // GB entries in flight: their slot loads, then their row reads, then the
     // entries applied one by one in list order (the same sums).  Two pay in
     // 2D; in 3D the extra registers cost a wave per SIMD.  LB: face centroids
-    // in flight in the limiter pass (measured best: 4 in 2D, 2 in 3D)
-    constexpr int GB = D == 2 ? 2 : 1, LB = D == 3 ? 2 : 4;
+    // in flight in the limiter pass (two measured best in 2D and 3D)
+    constexpr int GB = D == 2 ? 2 : 1, LB = 2;
     for (int e0 = 0; e0 < deg; e0 += GB) {
         SlotEnt<D> cur[GB];
         double rb[GB][XPW];
