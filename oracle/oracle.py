@@ -11,7 +11,10 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libphd_oracle.so")
+# PHD_ORACLE_LIB names another build of phd_oracle.c for the serial oracle:
+# tests/test_oracle_mutations.py points it at copies with a planted mistake
+# to show that the pins catch it.
+LIB_PATH = os.environ.get("PHD_ORACLE_LIB") or os.path.join(HERE, "libphd_oracle.so")
 OMP_PATH = os.path.join(HERE, "libphd_oracle_omp.so")
 
 _lib = None
@@ -175,6 +178,18 @@ def get_waves(dl, ul, pl, dr, ur, pr, gamma):
     lib().orc_get_waves(*[ctypes.c_double(v) for v in (dl, ul, pl, dr, ur, pr, gamma)],
                         ctypes.byref(sl), ctypes.byref(sc), ctypes.byref(sr))
     return sl.value, sc.value, sr.value
+
+
+def get_waves_pstar(dl, ul, pl, dr, ur, pr, gamma):
+    """get_waves plus the star-pressure estimate its wave speeds are built
+    from and the estimate used: 0 PVRS, 1 TRRS, 2 TSRS.  Returns
+    (sl, sc, sr, p_star, branch)."""
+    sl, sc, sr, ps = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    br = ctypes.c_int()
+    lib().orc_get_waves_pstar(*[ctypes.c_double(v) for v in (dl, ul, pl, dr, ur, pr, gamma)],
+                              ctypes.byref(sl), ctypes.byref(sc), ctypes.byref(sr),
+                              ctypes.byref(ps), ctypes.byref(br))
+    return sl.value, sc.value, sr.value, ps.value, br.value
 
 
 def update_from_fluxes(a, flux, dt, cons=None):

@@ -27,9 +27,12 @@
  * here: gcc folds pow(x, 2.0) to x*x at -O2, so the reference binary
  * computes the same correctly rounded product.
  *
- * Parity pinning of each function is listed in the test suite
- * (tests/test_oracle.py): known answers from the reference tests, closed
- * forms, invariants and brute force on tiny inputs.
+ * Parity pinning of each function is in the test suite
+ * (tests/test_oracle_riemann.py, tests/test_oracle_cells.py): known answers
+ * from the reference tests, closed forms, conservation identities (the
+ * Rankine-Hugoniot conditions of the HLLC star states), the exact solution
+ * where an estimate is exact, hand-worked cases and brute force on tiny
+ * inputs.
  */
 #include <math.h>
 #include <stdint.h>
@@ -490,9 +493,13 @@ void orc_add_temporal(int dim, int nf, double gamma, double dt,
 }
 
 /* ------------------------------------------------------------------------ */
-/* HLL.get_waves (riemann.pyx:375-504): Toro PVRS / TRRS / TSRS estimates */
-void orc_get_waves(double dl, double ul, double pl, double dr, double ur,
-                   double pr, double gamma, double* sl, double* sc, double* sr)
+/* HLL.get_waves (riemann.pyx:375-504): Toro PVRS / TRRS / TSRS estimates.
+ * get_waves_est also returns the star-pressure estimate the wave speeds are
+ * built from and the branch that produced it (0 PVRS, 1 TRRS, 2 TSRS), which
+ * the reference keeps local; orc_get_waves_pstar exposes them to the tests. */
+static void get_waves_est(double dl, double ul, double pl, double dr, double ur,
+                          double pr, double gamma, double* sl, double* sc,
+                          double* sr, double* p_est, int* branch)
 {
     const double Q = 2., c_floor = 1.0E-10;
     double cl = fmax(sqrt(gamma*pl/dl), c_floor);
@@ -505,7 +512,9 @@ void orc_get_waves(double dl, double ul, double pl, double dr, double ur,
 
     if (((p_max/p_min) < Q) && ((p_min < p_star) && (p_star < p_max))) {
         /* PVRS; u_star (445) is not used further */
+        *branch = 0;
     } else if (p_star <= p_min) {
+        *branch = 1;
         double z = (gamma - 1.)/(2.*gamma);
         double plr = pow(pl/pr, z);
         double u_star = (plr*ul/cl + ur/cr + 2.*(plr - 1.)/(gamma - 1.))/
@@ -513,6 +522,7 @@ void orc_get_waves(double dl, double ul, double pl, double dr, double ur,
         p_star  = .5*pl*pow(1. + (gamma - 1.)*(ul - u_star)/(2.*cl), 1./z);
         p_star += .5*pr*pow(1. + (gamma - 1.)*(u_star - ur)/(2.*cr), 1./z);
     } else {
+        *branch = 2;
         double Al = 2./((gamma + 1.)*dl);
         double Ar = 2./((gamma + 1.)*dr);
         double Bl = pl*((gamma - 1.)/(gamma + 1.));
@@ -530,6 +540,22 @@ void orc_get_waves(double dl, double ul, double pl, double dr, double ur,
     *sc = (pr - pl + dl*ul*(_sl - ul) - dr*ur*(_sr - ur))/(dl*(_sl - ul) - dr*(_sr - ur));
     *sl = _sl;
     *sr = _sr;
+    *p_est = p_star;
+}
+
+void orc_get_waves(double dl, double ul, double pl, double dr, double ur,
+                   double pr, double gamma, double* sl, double* sc, double* sr)
+{
+    double p_est;
+    int branch;
+    get_waves_est(dl, ul, pl, dr, ur, pr, gamma, sl, sc, sr, &p_est, &branch);
+}
+
+void orc_get_waves_pstar(double dl, double ul, double pl, double dr, double ur,
+                         double pr, double gamma, double* sl, double* sc,
+                         double* sr, double* p_est, int* branch)
+{
+    get_waves_est(dl, ul, pl, dr, ur, pr, gamma, sl, sc, sr, p_est, branch);
 }
 
 /* RiemannBase.deboost (riemann.pyx:190-222), Pakmor & Springel (2011) eq 17 */

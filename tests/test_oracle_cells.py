@@ -222,6 +222,55 @@ def test_gizmo_limiter_bounds():
         assert np.all(pr[q] >= lo - 1e-14) and np.all(pr[q] <= hi + 1e-14)
 
 
+def _two_cell_face(prim_i, prim_j, grad_i, grad_j, dcom_i, dcom_j, com):
+    """One 2D face between a cell at the origin and one at (1, 0)."""
+    a = {"dim": 2, "nf": 1, "n": 2,
+         "prim": [np.array([prim_i[q], prim_j[q]]) for q in range(4)],
+         "x": [np.array([0.0, 1.0]), np.array([0.0, 0.0])],
+         "dcom": [np.array([dcom_i[k], dcom_j[k]]) for k in range(2)],
+         "com": [np.array([com[k]]) for k in range(2)],
+         "fv": [np.zeros(1), np.zeros(1)],
+         "pair_i": np.array([0], dtype=np.int32), "pair_j": np.array([1], dtype=np.int32)}
+    grad = [np.array([grad_i[g], grad_j[g]]) for g in range(8)]
+    pl, pr = oracle.add_spatial(a, grad, boost=False, gizmo=True)
+    return [float(x[0]) for x in pl], [float(x[0]) for x in pr]
+
+
+def test_gizmo_limiter_hand_worked_case():
+    """Hopkins (2015) appendix B4 worked by hand on one face whose cell
+    centroids sit at unequal distances from the face centroid: generators at
+    x = 0 and 1 (|x_j - x_i| = 1), centroids at x = -0.25 and 1, face centroid
+    at x = 0.5, so |sep_i| = 0.75 and |sep_j| = 0.5.  psi1 = 1/2, psi2 = 1/4;
+    phibar_l = phi_l + |sep_i|/|x_j - x_i| (phi_r - phi_l) and its mirror.
+    Every value below is exact in binary except the one fallback quotient."""
+    prim_i = [1.0, -1.0, 0.5, 3.0]
+    prim_j = [2.0, -0.25, 0.5, 1.0]
+    # per field (d/dx, d/dy): density 2, vx 2 / -2, vy 1, pressure -4 / -6
+    grad_i = [2.0, 0.0, 2.0, 0.0, 1.0, 0.0, -4.0, 0.0]
+    grad_j = [2.0, 0.0, -2.0, 0.0, 1.0, 0.0, -6.0, 0.0]
+    L, R = _two_cell_face(prim_i, prim_j, grad_i, grad_j, (-0.25, 0.0), (0.0, 0.0), (0.5, 0.0))
+    # density (phi_i < phi_j): delta1 = 1/2, delta2 = 1/4, phibar_l = 1.75,
+    # phibar_r = 1.5; extrapolations 2.5 and 1.0 are cut to phibar -/+ delta2
+    assert L[0] == 2.0 and R[0] == 1.25
+    # vx (phi_i < phi_j, both negative): delta1 = 3/8, delta2 = 3/16;
+    # phibar_l = -7/16 and the left extrapolation 0.5 is cut to
+    # phibar_l + delta2 = -1/4; phi_max + delta1 changes sign, so
+    # phi_plus = phi_max / (1 + delta1/|phi_max|) and caps the right
+    # extrapolation 0.75
+    assert L[1] == -0.25 and R[1] == -0.25 / 2.5
+    # vy equal on both sides: the unextrapolated states
+    assert L[2] == 0.5 and R[2] == 0.5
+    # pressure (phi_i > phi_j): delta1 = 1, delta2 = 1/2, phibar_l = 1.5,
+    # phibar_r = 2; extrapolations 0 and 4 are cut to phibar_l - delta2 and
+    # phibar_r + delta2
+    assert L[3] == 1.0 and R[3] == 2.5
+    # the same face with the roles of the cells swapped (centroid offsets
+    # swapped with them) mirrors the result
+    L2, R2 = _two_cell_face(prim_j, prim_i, [-g for g in grad_j], [-g for g in grad_i],
+                            (0.0, 0.0), (0.25, 0.0), (0.5, 0.0))
+    assert L2 == R and R2 == L
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_temporal_is_primitive_euler_predictor(dim):
     """L = W - dt * sum_k A_k(W) dW/dx_k with the face-frame velocity in the
