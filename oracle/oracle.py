@@ -226,6 +226,37 @@ def core_step(a, dt, solver=1, boost=True, gizmo=True, limiter=0, omp=False):
             "bad": bad.value}
 
 
+def ghost_refresh_eos(a, cons, omp=False):
+    """Static-mesh epilogue (orc_ghost_refresh_eos) on copies of the conserved
+    fields `cons`; returns (cons, prim)."""
+    dim, n = a["dim"], a["n"]
+    cons = _f64([c.copy() for c in cons])
+    prim = [np.zeros(n) for _ in range(dim + 2)]
+    lo = np.array(a["lo"] + [0.0] * (3 - dim))
+    hi = np.array(a["hi"] + [0.0] * (3 - dim))
+    lib(omp).orc_ghost_refresh_eos(dim, n, ctypes.c_double(a["gamma"]), _p(a["tag"]),
+                                   _p(a["type"]), _p(a["map"]), _pp(_f64(a["x"])), _p(lo), _p(hi),
+                                   int(a["boundary"]), _p(a["volume"]), _pp(cons), _pp(prim))
+    return cons, prim
+
+
+def static_rollout(a, dts, solver=1, boost=True, gizmo=True, limiter=0, omp=False):
+    """StaticMeshMUSCLHancock steps (integrate.py:349-378) with the given
+    time steps: core step, then ghost refresh + EOS.  `a` holds the starting
+    state (prim and cons consistent, ghosts refreshed) and is not changed.
+    Returns (cons, prim, rc, bad, steps done)."""
+    b = dict(a)
+    b["prim"] = _f64([x.copy() for x in a["prim"]])
+    b["cons"] = _f64([x.copy() for x in a["cons"]])
+    for s, dt in enumerate(dts):
+        r = core_step(b, float(dt), solver=solver, boost=boost, gizmo=gizmo, limiter=limiter,
+                      omp=omp)
+        if r["rc"] != 0:
+            return r["cons"], b["prim"], r["rc"], r["bad"], s
+        b["cons"], b["prim"] = ghost_refresh_eos(b, r["cons"], omp=omp)
+    return b["cons"], b["prim"], 0, -1, len(dts)
+
+
 DUAL_PATH = os.path.join(HERE, "libphd_oracle_dual.so")
 _dual = None
 

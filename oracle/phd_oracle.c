@@ -338,6 +338,36 @@ void orc_ghost_gradients(int dim, int n, const int8_t* tag, const int8_t* type,
     }
 }
 
+/* Static-mesh epilogue of StaticMeshMUSCLHancock.evolve_timestep
+ * (integrate.py:371-377): DomainManager.update_ghost_fields(particles,
+ * conservative, True), serial branch (domain_manager.pyx:757-774): every
+ * ghost takes its image's conserved fields, then the boundary rule
+ * (Reflective.update_fields, boundary.pyx:309-348: momentum and velocity of
+ * exterior cells negated on each axis where they lie outside the box;
+ * Periodic: none), then IdealGas.primitive_from_conservative over all cells
+ * (equation_state.pyx:81-117). */
+void orc_ghost_refresh_eos(int dim, int n, double gamma, const int8_t* tag,
+                           const int8_t* type, const int32_t* map,
+                           double* const* x, const double* lo,
+                           const double* hi, int boundary, const double* vol,
+                           double* const* cons, double* const* prim)
+{
+    for (int i = 0; i < n; i++)
+        if (tag[i] == GHOST)
+            for (int q = 0; q < dim + 2; q++)
+                cons[q][i] = cons[q][map[i]];
+    if (boundary == 0) {
+        for (int i = 0; i < n; i++) {
+            if (type[i] != EXTERIOR) continue;
+            for (int k = 0; k < dim; k++) {
+                if (x[k][i] < lo[k]) { prim[1+k][i] *= -1; cons[1+k][i] *= -1; }
+                if (x[k][i] > hi[k]) { prim[1+k][i] *= -1; cons[1+k][i] *= -1; }
+            }
+        }
+    }
+    orc_prim_from_cons(dim, n, gamma, cons, vol, prim);
+}
+
 /* ------------------------------------------------------------------------ */
 /* PieceWiseConstant.compute_states (reconstruction.pyx:254-352) */
 void orc_states_constant(int dim, int nf, double* const* prim, double* const* fv,

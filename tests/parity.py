@@ -29,7 +29,11 @@ def rel_check(got, want, rtol=RTOL, floor_frac=1e-6, name=""):
     return float(np.mean(got == want))
 
 
-def check_fields(got_list, want_list, name, mask=None, **kw):
+def check_fields(got_list, want_list, name, mask=None, bitwise=False, **kw):
+    """rel_check per field; with bitwise=True every entry must also be
+    bit-identical (checked after the tolerance, so a failure reports the
+    worst relative difference first).  Returns the smallest bit-identical
+    fraction."""
     fr = []
     for q, (g, w) in enumerate(zip(got_list, want_list)):
         g = np.asarray(g)
@@ -37,4 +41,20 @@ def check_fields(got_list, want_list, name, mask=None, **kw):
         if mask is not None:
             g, w = g[mask], w[mask]
         fr.append(rel_check(g, w, name="%s[%d]" % (name, q), **kw))
+        if bitwise:
+            assert_bitwise(g, w, "%s[%d]" % (name, q))
     return min(fr) if fr else 1.0
+
+
+def assert_bitwise(got, want, name=""):
+    """Exact equality of two float64 arrays (NaNs equal when both are NaN);
+    reports the first differing entry and the count."""
+    got = np.asarray(got, dtype=np.float64)
+    want = np.asarray(want, dtype=np.float64)
+    assert got.shape == want.shape, (name, got.shape, want.shape)
+    same = (got == want) | (np.isnan(got) & np.isnan(want))
+    if not np.all(same):
+        bad = np.nonzero(~same)[0]
+        i = int(bad[0])
+        raise AssertionError("%s: %d of %d entries differ; first at %d: got %r want %r" % (
+            name, bad.size, got.size, i, got[i], want[i]))

@@ -3,8 +3,11 @@ ABI, on the three synthetic configurations at oracle-sized meshes.
 
 Each stage is fed the same inputs on both sides (the oracle gets host copies
 of the device arrays) and compared with the tolerance of tests/parity.py;
-BITWISE kernels are expected to agree exactly except where libm pow() and
-the GPU's pow() (or its series for bases near one) round differently.
+BITWISE kernels agree exactly except where libm pow() and the GPU's pow()
+(or its series for bases near one) round differently: on these mesh
+problems no stage meets such a difference, and every stage is asserted bit
+for bit (bitwise=True); the random-state Riemann test, which does meet
+them, is held to the tolerance.
 """
 import numpy as np
 import pytest
@@ -40,10 +43,10 @@ def test_stage_parity(gpu, name, n):
 
     # EOS both ways (equation_state.pyx:43-117)
     cons_ref = oracle.cons_from_prim(dim, a0["gamma"], a0["prim"], a0["volume"])
-    check_fields(_np(pc.group("conservative")), cons_ref, "cons_from_prim")
+    check_fields(_np(pc.group("conservative")), cons_ref, "cons_from_prim", bitwise=True)
     prim_ref = oracle.prim_from_cons(dim, a0["gamma"], a0["cons"], a0["volume"])
     p.eos.primitive_from_conservative(pc)
-    check_fields(_np(pc.group("primitive")), prim_ref, "prim_from_cons")
+    check_fields(_np(pc.group("primitive")), prim_ref, "prim_from_cons", bitwise=True)
     a0 = problems.host_arrays(p)
 
     # CFL (riemann.pyx:120-188)
@@ -51,16 +54,17 @@ def test_stage_parity(gpu, name, n):
     dt_ref = oracle.time_step(dim, a0["gamma"], p.riemann.cfl, p.riemann.boost, a0["tag"],
                               a0["prim"], a0["volume"])
     rel_check(np.array([dt]), np.array([dt_ref]), name="dt")
+    assert dt == dt_ref
 
     # generator + face velocities (mesh.pyx:363-477)
     p.mesh.assign_generator_velocities(pc, p.eos)
     w_ref = oracle.generator_velocities(dim, a0["gamma"], mesh.eta, mesh.regularize,
                                         a0["prim"], a0["dcom"], a0["volume"])
-    check_fields(_np(pc.group("w")), w_ref, "w")
+    check_fields(_np(pc.group("w")), w_ref, "w", bitwise=True)
     p.mesh.assign_face_velocities(pc)
     a = problems.host_arrays(p)
     fv_ref = oracle.face_velocities(dim, a["x"], a["w"], a["com"], a["pair_i"], a["pair_j"])
-    check_fields(a["fv"], fv_ref, "face velocity")
+    check_fields(a["fv"], fv_ref, "face velocity", bitwise=True)
 
     # gradients + limiter + ghost gradients (reconstruction.pyx:563-739)
     recon = p.recon
@@ -69,7 +73,7 @@ def test_stage_parity(gpu, name, n):
     assert rc == 0
     oracle.ghost_gradients(a, g_ref)
     g_gpu = recon.grad.rows().cpu().numpy()
-    fr = check_fields([g_gpu[:, q] for q in range(len(g_ref))], g_ref, "grad")
+    fr = check_fields([g_gpu[:, q] for q in range(len(g_ref))], g_ref, "grad", bitwise=True)
     print("grad bitwise fraction", fr)
 
     # materialised states (reconstruction.pyx:741-1072) at dt/2
@@ -79,25 +83,27 @@ def test_stage_parity(gpu, name, n):
     R = _np(recon.right_states.group("primitive"))
     pl, pr = oracle.add_spatial(a, g_ref, boost=True, gizmo=True)
     oracle.add_temporal(a, g_ref, half, pl, pr, boost=True)
-    check_fields(L, pl, "left states")
-    check_fields(R, pr, "right states")
+    check_fields(L, pl, "left states", bitwise=True)
+    check_fields(R, pr, "right states", bitwise=True)
 
     # Riemann from materialised states (riemann.pyx:74-98, 521-682, 190-222)
     p.riemann.compute_fluxes(pc, mesh, recon, p.eos)
     F_ref, _, _ = oracle.riemann("hllc", dim, a["gamma"], True, pl, pr, a["normal"], a["fv"])
-    fr = check_fields(_np(p.riemann.fluxes.group("conservative")), F_ref, "flux (states)")
+    fr = check_fields(_np(p.riemann.fluxes.group("conservative")), F_ref, "flux (states)",
+                      bitwise=True)
     print("flux bitwise fraction", fr)
 
     # fused states + Riemann
     recon.compute_states(pc, mesh, p.eos.get_gamma(), p.domain_manager, half, True)
     p.riemann.compute_fluxes(pc, mesh, recon, p.eos)
-    check_fields(_np(p.riemann.fluxes.group("conservative")), F_ref, "flux (fused)")
+    check_fields(_np(p.riemann.fluxes.group("conservative")), F_ref, "flux (fused)", bitwise=True)
 
     # conserved update (mesh.pyx:479-551)
     cons_ref, rc, _ = oracle.update_from_fluxes(a, F_ref, dt)
     assert rc == 0
     mesh.update_from_fluxes(pc, p.riemann, dt)
-    check_fields(_np(pc.group("conservative")), cons_ref, "cons update", mask=real)
+    check_fields(_np(pc.group("conservative")), cons_ref, "cons update", mask=real,
+                 bitwise=True)
 
 
 @pytest.mark.parametrize("name,n", CASES)
@@ -115,8 +121,10 @@ def test_core_step_parity(gpu, name, n):
     step.run(dt)
     step.check()
     real = a["tag"] == 0
-    check_fields(_np(p.riemann.fluxes.group("conservative")), ref["flux"], "core flux")
-    check_fields(_np(p.particles.group("conservative")), ref["cons"], "core cons", mask=real)
+    check_fields(_np(p.riemann.fluxes.group("conservative")), ref["flux"], "core flux",
+                 bitwise=True)
+    check_fields(_np(p.particles.group("conservative")), ref["cons"], "core cons", mask=real,
+                 bitwise=True)
 
 
 @pytest.mark.parametrize("solver", ["hll", "hllc", "exact"])

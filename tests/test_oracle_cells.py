@@ -172,6 +172,46 @@ def test_ghost_gradient_flip_rule():
         np.testing.assert_array_equal(grad[g], w)
 
 
+def test_ghost_refresh_eos_hand_placed():
+    """domain_manager.pyx:757-774 + boundary.pyx:336-348 + the EOS: each
+    ghost takes its image's conserved values, momentum (and velocity) flip on
+    each axis where an exterior ghost is outside the box -- twice for a
+    corner ghost -- and the primitives follow from the conserved fields
+    (checked against the closed-form EOS on exact binary values)."""
+    dim, n = 2, 6
+    x = [np.array([0.25, 0.75, -0.25, 0.75, -0.25, 0.5]),
+         np.array([0.25, 0.5, 0.25, 1.5, -0.5, 0.5])]
+    tag = np.array([0, 0, 1, 1, 1, 1], np.int8)
+    typ = np.array([4, 4, 2, 2, 2, 2], np.int8)
+    mp = np.array([0, 0, 0, 1, 0, 1], np.int32)
+    vol = np.array([0.25, 0.5, 0.25, 0.5, 0.25, 0.5])
+    cons = [np.array([0.5, 1.0, 9, 9, 9, 9.0]),          # mass
+            np.array([0.25, -0.5, 9, 9, 9, 9.0]),        # momentum-x
+            np.array([-0.125, 1.0, 9, 9, 9, 9.0]),       # momentum-y
+            np.array([2.0, 4.0, 9, 9, 9, 9.0])]          # energy
+    a = {"dim": dim, "n": n, "tag": tag, "type": typ, "map": mp, "x": x, "volume": vol,
+         "lo": [0.0, 0.0], "hi": [1.0, 1.0], "boundary": 0, "gamma": 1.4}
+    c, p = oracle.ghost_refresh_eos(a, cons)
+    sign = {2: (-1.0, 1.0), 3: (1.0, -1.0), 4: (-1.0, -1.0), 5: (1.0, 1.0)}
+    for i in range(2, 6):
+        s = mp[i]
+        assert c[0][i] == cons[0][s] and c[3][i] == cons[3][s]
+        assert c[1][i] == sign[i][0] * cons[1][s] and c[2][i] == sign[i][1] * cons[2][s]
+    for i in range(n):
+        d = c[0][i] / vol[i]
+        v = [c[1][i] / c[0][i], c[2][i] / c[0][i]]
+        pres = (c[3][i] / vol[i] - 0.5 * d * (v[0] * v[0] + v[1] * v[1])) * 0.4
+        assert p[0][i] == d and p[1][i] == v[0] and p[2][i] == v[1]
+        assert abs(p[3][i] - pres) <= 1e-15 * abs(pres)
+    # cell 0: rho 2, v (0.5, -0.25), E/V 8 -> p = (8 - 0.3125) * 0.4
+    assert p[0][0] == 2.0 and p[1][0] == 0.5 and p[2][0] == -0.25
+    assert abs(p[3][0] - 3.075) < 1e-15
+    a["boundary"] = 1
+    c, p = oracle.ghost_refresh_eos(a, cons)
+    for q in range(4):
+        np.testing.assert_array_equal(c[q][2:], cons[q][mp[2:]])
+
+
 def test_constant_state_reconstruction_fixture():
     v = GOLD["constant_state_reconstruction"]["value"]
     a = lattice(8, 2, seed=0)

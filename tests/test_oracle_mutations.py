@@ -58,6 +58,9 @@ MUTATIONS = {
     # exact solver fan, riemann.pyx:800-830
     "exact_fan": [("double c = (2.0/(gamma + 1.0))*(cl + 0.5*(gamma - 1.0)*vnl);",
                    "double c = (2.0/(gamma + 1.0))*(cl - 0.5*(gamma - 1.0)*vnl);")],
+    # static-mesh epilogue, boundary.pyx:336-348
+    "ghost_refresh_momentum": [("if (x[k][i] < lo[k]) { prim[1+k][i] *= -1; cons[1+k][i] *= -1; }",
+                                "if (x[k][i] < lo[k]) { prim[1+k][i] *= -1; }")],
     # face velocities, mesh.pyx:470-477
     "face_velocity_sign": [("fv[k][m] = 0.5*(wx[k][i] + wx[k][j]) + factor*(x[k][j] - x[k][i]);",
                             "fv[k][m] = 0.5*(wx[k][i] + wx[k][j]) - factor*(x[k][j] - x[k][i]);")],
