@@ -173,7 +173,8 @@ int phd_compute_gradients(phd_handle* h, phd_plan* p, const phd_cells* c,
                           int limiter, double* rec, int32_t rec_ld);
 /* DomainManager.update_ghost_gradients (serial), domain_manager.pyx:819-837,
  * with Reflective.update_gradients (boundary.pyx:265-307) or the periodic
- * no-op (boundary.pyx:457-478).  lo/hi: host arrays of dim box bounds. */
+ * no-op (boundary.pyx:457-478).  lo/hi: host arrays of dim box bounds.
+ * PHD_E_ARG when the plan lists ghosts and c->map is NULL. */
 int phd_update_ghost_gradients(phd_handle* h, phd_plan* p, const phd_cells* c,
                                int boundary, const double* lo, const double* hi,
                                double* rec, int32_t rec_ld);
@@ -245,10 +246,42 @@ int phd_core_step(phd_handle* h, phd_plan* p, const phd_cells* c,
  * followed by IdealGas.primitive_from_conservative (equation_state.pyx:81-117)
  * over all cells, fused (StaticMeshMUSCLHancock.evolve_timestep,
  * integrate.py:371-377).  With a plan, also refreshes its packed cell rows
- * (then phd_core_step may be called with PHD_CORE_XP_FRESH). */
+ * (then phd_core_step may be called with PHD_CORE_XP_FRESH).  c->map may be
+ * NULL only when there are no ghosts: PHD_E_ARG when the plan lists ghosts,
+ * and without a plan the tags are scanned first (synchronising). */
 int phd_ghost_refresh_eos(phd_handle* h, phd_plan* p, const phd_cells* c,
                           int boundary, const double* lo, const double* hi,
                           double gamma);
+
+/* ---- domain and boundary loops between flux steps ------------------------
+ * (domain/domain_manager.pyx, domain/boundary.pyx; serial branches) */
+/* DomainManager.move_generators, domain_manager.pyx:566-590: x += dt*w for
+ * real cells.  Writes the cells' position arrays (c->x). */
+int phd_move_generators(phd_handle* h, const phd_cells* c, double dt);
+/* Periodic.migrate_particles, boundary.pyx:502-538: each real cell outside
+ * the box [lo, hi] (host arrays of dim bounds) is wrapped on every axis
+ * where its position before the wrap is <= lo (+ (hi - lo)) or >= hi
+ * (- (hi - lo)).  Writes c->x. */
+int phd_migrate_periodic(phd_handle* h, const phd_cells* c, const double* lo,
+                         const double* hi);
+/* Reflective.migrate_particles, boundary.pyx:236-263 (the check that raises
+ * "particle left domain in reflective boundary condition!!"): *first = the
+ * smallest index of a real cell outside the box, or -1.  Synchronising. */
+int phd_first_outside(phd_handle* h, const phd_cells* c, const double* lo,
+                      const double* hi, int64_t* first);
+/* Reflective.update_fields, boundary.pyx:309-348: velocity (primitive
+ * group) and momentum (conservative group) of every exterior cell (type 2)
+ * negated on each axis where it lies outside the box. */
+int phd_reflect_fields(phd_handle* h, const phd_cells* c, const double* lo,
+                       const double* hi);
+/* DomainManager.update_ghost_fields, serial branch, domain_manager.pyx:
+ * 757-770: every ghost (tag 1) takes each of the n_fields arrays' value at
+ * its image c->map[i].  fields: device arrays of n_cells elements of
+ * elem_size bytes (1, 4 or 8) each; the boundary rule is the caller's
+ * separate phd_reflect_fields, as in the reference. */
+int phd_update_ghost_fields(phd_handle* h, const phd_cells* c,
+                            void* const* fields, const int32_t* elem_size,
+                            int32_t n_fields);
 
 /* ---- batched ensembles ---------------------------------------------------
  * No counterpart in the reference, which runs one problem per process.

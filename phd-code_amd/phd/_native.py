@@ -119,6 +119,11 @@ _FLUX_SIGS = {
     "phd_face_summands": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_dbl, c_dbl, c_i32, c_i32,
                            c_i32, c_vp], c_i32),
     "phd_update_from_summands": ([c_vp, c_vp, c_vp], c_i32),
+    "phd_move_generators": ([c_vp, c_vp, c_dbl], c_i32),
+    "phd_migrate_periodic": ([c_vp, c_vp, c_vp, c_vp], c_i32),
+    "phd_first_outside": ([c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_i64)], c_i32),
+    "phd_reflect_fields": ([c_vp, c_vp, c_vp, c_vp], c_i32),
+    "phd_update_ghost_fields": ([c_vp, c_vp, c_vp, c_vp, c_i32], c_i32),
 }
 
 # symbols added by later parts of the library register here too

@@ -11,11 +11,18 @@
 
 #define PHD_BLOCK 256
 
+// handle scratch words: [0] update key, [1] neighbor cell, [2] exact face,
+// [3] dt bits, [4] first real cell outside the box, [5] first ghost cell;
+// [6..7] spare
+#define PHD_ERR_SLOTS 8
+#define PHD_ERR_SLOT_OUTSIDE 4
+#define PHD_ERR_SLOT_GHOST 5
+
 struct phd_handle {
     int device;
     hipStream_t stream;
     int mode;
-    unsigned long long* d_err;   // [0] update key, [1] neighbor cell, [2] exact face, [3] dt bits
+    unsigned long long* d_err;   // PHD_ERR_SLOTS words
     unsigned long long* h_err;   // pinned host mirror
 };
 
@@ -81,8 +88,9 @@ template <int D> struct XpLay {
 
 struct FaceArrays { double* q[5]; };
 
-// doubles per summand row: D+2 padded to even (16-byte aligned rows)
-template <int D> constexpr int summand_width() { return (D + 3) & ~1; }
+// doubles per summand row: the D+2 conserved components, unpadded (32 B in
+// 2D, moved as 16-byte pairs; 40 B in 3D, moved as single doubles)
+template <int D> constexpr int summand_width() { return D + 2; }
 
 template <int D> struct Lay {
     static constexpr int NF = D + 2;                 // primitive fields
@@ -120,6 +128,77 @@ inline unsigned sched_grid(long long n)
 
 __device__ __forceinline__ long long xcd_index(long long n) { return sched_index<PHD_BLOCK, 1>(n); }
 
+// cell radius (riemann.pyx:155-158, mesh.pyx:414-417); pi is numpy.pi
+template <int D>
+__device__ __forceinline__ double cell_radius(double vol)
+{
+    if (D == 2) return sqrt(vol / 3.141592653589793);
+    return pow(3.0 * vol / (4.0 * 3.141592653589793), 1.0 / 3.0);
+}
+
+// CFL time step (riemann.pyx:120-188).  min is order independent: a
+// wavefront min and an atomic min on the bit pattern (non-negative doubles
+// order as unsigned integers) give the exact, deterministic minimum.
+// cfl_value: the bound of one real cell; +inf for NaN, which fmin would
+// otherwise skip
+template <int D>
+__device__ __forceinline__ double cfl_value(const phd_cells& c, long long i, double gamma, int boost)
+{
+    double v;
+    double d = c.prim[0][i], p = c.prim[D + 1][i];
+    double cs = sqrt(gamma * p / d);
+    double R = cell_radius<D>(c.volume[i]);
+    if (boost) {
+        v = R / cs;
+    } else {
+        double vsq = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) { double u = c.prim[1 + k][i]; vsq += u * u; }
+        v = R / (cs + sqrt(vsq));
+    }
+    if (!(v >= 0.0)) v = __longlong_as_double((long long)INF_BITS);
+    return v;
+}
+
+__global__ void k_fill_u64(unsigned long long* a, int n, unsigned long long v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+template <int D>
+__device__ __forceinline__ void store_summand(double* g, long long m, const double* row)
+{
+    constexpr int GW = summand_width<D>();
+    double* o = g + (size_t)m * GW;
+    if (GW % 2 == 0) {
+#pragma unroll
+        for (int q = 0; q < GW / 2; q++)
+            reinterpret_cast<double2*>(o)[q] = make_double2(row[2 * q], row[2 * q + 1]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < GW; q++) o[q] = row[q];
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void load_summand(const double* g, long long m, double* s)
+{
+    constexpr int GW = summand_width<D>();
+    const double* r = g + (size_t)m * GW;
+    if (GW % 2 == 0) {
+#pragma unroll
+        for (int q = 0; q < GW / 2; q++) {
+            const double2 v = reinterpret_cast<const double2*>(r)[q];
+            s[2 * q] = v.x;
+            s[2 * q + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < GW; q++) s[q] = r[q];
+    }
+}
+
 __device__ __forceinline__ void ld2(const double* p, double& a, double& b)
 {
     double2 v = *reinterpret_cast<const double2*>(p);
@@ -150,6 +229,29 @@ __device__ __forceinline__ void load_rec(const double* rec, int ld, int i, doubl
     for (int q = 0; q < Y::LD / 2; q++) ld2(p + 2 * q, row[2 * q], row[2 * q + 1]);
 }
 
+// left/right states of a face from its two cell-record rows ri, rj (in
+// registers, or ri in LDS): spatial extrapolation + GIZMO, then the
+// half-step predictor
+template <int D>
+__device__ __forceinline__ void face_states_rows(const double* ri, const double* rj,
+                                                 const double* com, double diff, const double* fv,
+                                                 double gamma, double dt, int boost, int gizmo,
+                                                 int stages, double* L, double* R)
+{
+    using Y = Lay<D>;
+    const double* pi = ri + Y::P0;
+    const double* pj = rj + Y::P0;
+    if (stages & PHD_STAGE_SPATIAL) {
+        boost_copy<D>(pi, pj, fv, boost, L, R);
+        spatial_states<D>(com, diff, ri + Y::C0, rj + Y::C0, pi, pj, ri + Y::G0, rj + Y::G0,
+                          gizmo, L, R);
+    }
+    if (stages & PHD_STAGE_TEMPORAL) {
+        temporal_side<D>(pi, ri + Y::G0, fv, boost, gamma, dt, L);
+        temporal_side<D>(pj, rj + Y::G0, fv, boost, gamma, dt, R);
+    }
+}
+
 // left/right states of face m into L, R (primitive group, face frame)
 template <int D, int ORDER>
 __device__ __forceinline__ void face_states(const phd_cells& c, const double* rec, int ld, int i,
@@ -168,17 +270,7 @@ __device__ __forceinline__ void face_states(const phd_cells& c, const double* re
     double ri[Y::LD], rj[Y::LD];
     load_rec<D>(rec, ld, i, ri);
     load_rec<D>(rec, ld, j, rj);
-    const double* pi = ri + Y::P0;
-    const double* pj = rj + Y::P0;
-    if (stages & PHD_STAGE_SPATIAL) {
-        boost_copy<D>(pi, pj, fv, boost, L, R);
-        spatial_states<D>(com, diff, ri + Y::C0, rj + Y::C0, pi, pj, ri + Y::G0, rj + Y::G0,
-                          gizmo, L, R);
-    }
-    if (stages & PHD_STAGE_TEMPORAL) {
-        temporal_side<D>(pi, ri + Y::G0, fv, boost, gamma, dt, L);
-        temporal_side<D>(pj, rj + Y::G0, fv, boost, gamma, dt, R);
-    }
+    face_states_rows<D>(ri, rj, com, diff, fv, gamma, dt, boost, gizmo, stages, L, R);
 }
 
 } // namespace

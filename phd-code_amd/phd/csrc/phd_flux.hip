@@ -17,8 +17,8 @@
 //                     windows of 256 cells)
 //     sliced lists    per slot: other cell, face id | incoming bit, face
 //                     centroid, area
-//     g[F][GW]        (dt*area)*flux per face, one row of D+2 doubles padded
-//                     to an even count, written by the face kernel
+//     g[F][D+2]       (dt*area)*flux per face, one unpadded row of D+2
+//                     doubles (32 B in 2D, 40 B in 3D), written by the face kernel
 //                     (coalesced) and applied by the update as -g to the
 //                     owner and +g to the neighbour
 //   Cell record rows rec[Na][ld] (the gradient container): primitive(D+2) |
@@ -52,6 +52,14 @@ __global__ void k_reset_err(unsigned long long* e)
 __global__ void k_set_err(unsigned long long* e, int slot, unsigned long long v)
 {
     if (threadIdx.x == 0) atomicMin(&e[slot], v);
+}
+
+// first cell with tag t into *first (atomic min)
+__global__ __launch_bounds__(PHD_BLOCK) void k_first_tag(int n, const int8_t* __restrict__ tag,
+                                                         int t, unsigned long long* first)
+{
+    const long long i = (long long)blockIdx.x * PHD_BLOCK + threadIdx.x;
+    if (i < n && tag[i] == t) atomicMin(first, (unsigned long long)i);
 }
 
 
@@ -91,38 +99,6 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_prim_from_cons(phd_cells c, doubl
         v_sq += v * v;
     }
     c.prim[D + 1][i] = (c.cons[D + 1][i] / vol - .5 * d * v_sq) * (gamma - 1.);
-}
-
-// cell radius (riemann.pyx:155-158, mesh.pyx:414-417); pi is numpy.pi
-template <int D>
-__device__ __forceinline__ double cell_radius(double vol)
-{
-    if (D == 2) return sqrt(vol / 3.141592653589793);
-    return pow(3.0 * vol / (4.0 * 3.141592653589793), 1.0 / 3.0);
-}
-
-// CFL time step (riemann.pyx:120-188).  min is order independent: a
-// wavefront min and an atomic min on the bit pattern (non-negative doubles
-// order as unsigned integers) give the exact, deterministic minimum.
-// cfl_value: the bound of one real cell; +inf for NaN, which fmin would
-// otherwise skip
-template <int D>
-__device__ __forceinline__ double cfl_value(const phd_cells& c, long long i, double gamma, int boost)
-{
-    double v;
-    double d = c.prim[0][i], p = c.prim[D + 1][i];
-    double cs = sqrt(gamma * p / d);
-    double R = cell_radius<D>(c.volume[i]);
-    if (boost) {
-        v = R / cs;
-    } else {
-        double vsq = 0.0;
-#pragma unroll
-        for (int k = 0; k < D; k++) { double u = c.prim[1 + k][i]; vsq += u * u; }
-        v = R / (cs + sqrt(vsq));
-    }
-    if (!(v >= 0.0)) v = __longlong_as_double((long long)INF_BITS);
-    return v;
 }
 
 template <int D>
@@ -616,8 +592,13 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
                     double dphi = 0.0;
 #pragma unroll
                     for (int k = 0; k < D; k++) dphi += df[n * D + k] * relb[b][k];
-                    if (dphi > 0) dpos[n] = fmax(dpos[n], dphi);
-                    else if (dphi < 0) dneg[n] = fmin(dneg[n], dphi);
+                    // dpos starts at 0 and only grows, dneg starts at 0 and
+                    // only shrinks, and fmax/fmin skip NaN: the unconditional
+                    // forms leave each one exactly as the reference's
+                    // "dphi > 0" / "dphi < 0" branches do (a -0 is never
+                    // taken as > 0 or < 0 below)
+                    dpos[n] = fmax(dpos[n], dphi);
+                    dneg[n] = fmin(dneg[n], dphi);
                 }
             }
         }
@@ -783,10 +764,8 @@ __global__ __launch_bounds__(BS) void k_face(phd_cells c, phd_faces f,
         dt_full = bdt[fprob[m]];
         dt_half = 0.5 * dt_full;
         if (dt_full == 0.0) {              // a frozen problem: no change
-            constexpr int GW = summand_width<D>();
-            double2* o = reinterpret_cast<double2*>(g + (size_t)m * GW);
-#pragma unroll
-            for (int q = 0; q < GW / 2; q++) o[q] = make_double2(0.0, 0.0);
+            const double zero[NF] = {};
+            store_summand<D>(g, m, zero);
             return;
         }
     }
@@ -805,14 +784,11 @@ __global__ __launch_bounds__(BS) void k_face(phd_cells c, phd_faces f,
         for (int q = 0; q < NF; q++) flux.q[q][m] = F[q];
     }
     if (OUT & 2) {
-        constexpr int GW = summand_width<D>();
         const double t = dt_full * area;
-        double row[GW];
+        double row[NF];
 #pragma unroll
-        for (int q = 0; q < GW; q++) row[q] = q < NF ? t * F[q] : 0.0;
-        double2* o = reinterpret_cast<double2*>(g + (size_t)m * GW);
-#pragma unroll
-        for (int q = 0; q < GW / 2; q++) o[q] = make_double2(row[2 * q], row[2 * q + 1]);
+        for (int q = 0; q < NF; q++) row[q] = t * F[q];
+        store_summand<D>(g, m, row);
     }
 }
 
@@ -895,19 +871,6 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_update(phd_cells c, const double*
 // incoming faces followed by one run of consecutive owned face ids
 // (owner_runs): the incoming rows are gathered, the owned run is read as
 // consecutive rows without touching the list again.
-template <int D>
-__device__ __forceinline__ void load_summand(const double* g, int fid, double* s)
-{
-    constexpr int GW = summand_width<D>();
-    const double2* r = reinterpret_cast<const double2*>(g + (size_t)fid * GW);
-#pragma unroll
-    for (int q = 0; q < GW / 2; q++) {
-        const double2 v = r[q];
-        s[2 * q] = v.x;
-        s[2 * q + 1] = v.y;
-    }
-}
-
 // Conserved update of the cell at sliced position t (cell i = perm[t]):
 // u = its conserved values plus its summand rows in list order.  False for a
 // ghost (nothing to write; a ghost with faces reports a fault).
@@ -1377,7 +1340,8 @@ int launch_ghost(phd_handle* h, phd_plan* p, const phd_cells* c, int boundary, c
     if (c->n_cells == 0) return 0;
     if (boundary != PHD_BOUNDARY_REFLECTIVE && boundary != PHD_BOUNDARY_PERIODIC) return PHD_E_MODE;
     if (boundary == PHD_BOUNDARY_REFLECTIVE && (!lo || !hi)) return PHD_E_ARG;
-    if (!c->map || p->n_ghosts == 0) return 0;
+    if (p->n_ghosts == 0) return 0;
+    if (!c->map) return PHD_E_ARG;          // ghosts need their images
     double l[3], u[3];
     bounds3(D, boundary, lo, hi, l, u);
     k_ghost_rec<D><<<xcd_grid(p->n_ghosts), PHD_BLOCK, 0, h->stream>>>(
@@ -1488,7 +1452,7 @@ int build_slots(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces*
     const size_t s_base = al(sizeof(int32_t) * (size_t)(ns + 1));
     const size_t s_j = al(sizeof(int32_t) * (size_t)total);
     const size_t s_d = al(sizeof(double) * (size_t)total);
-    const size_t s_g = al(sizeof(double) * (size_t)(NF + (NF & 1)) * (size_t)nf);
+    const size_t s_g = al(sizeof(double) * (size_t)NF * (size_t)nf);   // summand_width
     const size_t s_ci = al(sizeof(int2) * (size_t)na);
     const size_t bytes = s_base + 2 * s_j + 4 * s_d + s_g + s_ci;
     if (hipMalloc(&p->slot_base, bytes) != hipSuccess) { free(hw); p->slot_base = nullptr; return PHD_E_HIP; }
@@ -1552,8 +1516,8 @@ int phd_handle_create(phd_handle** out, int device)
     h->device = device;
     h->stream = nullptr;
     h->mode = PHD_MODE_BITWISE;
-    if (hipMalloc(&h->d_err, 4 * sizeof(unsigned long long)) != hipSuccess) { delete h; return PHD_E_HIP; }
-    if (hipHostMalloc(&h->h_err, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+    if (hipMalloc(&h->d_err, PHD_ERR_SLOTS * sizeof(unsigned long long)) != hipSuccess) { delete h; return PHD_E_HIP; }
+    if (hipHostMalloc(&h->h_err, PHD_ERR_SLOTS * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
         (void)hipFree(h->d_err); delete h; return PHD_E_HIP;
     }
     k_reset_err<<<1, 64, 0, h->stream>>>(h->d_err);
@@ -1922,6 +1886,19 @@ int phd_ghost_refresh_eos(phd_handle* h, phd_plan* p, const phd_cells* c, int bo
     const int refl = boundary == PHD_BOUNDARY_REFLECTIVE;
     double* xp = p ? p->xp : nullptr;
     if (!c->map) {
+        // no images: valid only without ghosts (the plan lists them; without
+        // a plan the tags are scanned, synchronously)
+        if (p && p->n_ghosts > 0) return PHD_E_ARG;
+        if (!p) {
+            unsigned long long* slot = h->d_err + PHD_ERR_SLOT_GHOST;
+            k_fill_u64<<<1, 64, 0, h->stream>>>(slot, 1, NOERR);
+            k_first_tag<<<nblocks(c->n_cells), PHD_BLOCK, 0, h->stream>>>(c->n_cells, c->tag, 1, slot);
+            LAUNCHCHK();
+            HIPCHK(hipMemcpyAsync(h->h_err + PHD_ERR_SLOT_GHOST, slot, sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            if (h->h_err[PHD_ERR_SLOT_GHOST] != NOERR) return PHD_E_ARG;
+        }
         int rc = phd_primitive_from_conservative(h, c, gamma);
         if (rc || !p) return rc;
         return phd_pack_cells(h, p, c);

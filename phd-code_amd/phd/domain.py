@@ -10,10 +10,7 @@ mesh builder (Mesh.build_geometry).
 """
 import ctypes
 
-import torch
-
 from . import _native as nat
-from .utils import ParticleTAGS
 from .views import cells_view
 
 
@@ -30,26 +27,25 @@ class BoundaryConditionBase(object):
         raise NotImplementedError("BoundaryBase::update_fields called!")
 
 
-def _outside_masks(particles, domain_manager):
-    dim = len(particles.carray_named_groups["position"])
-    ext = particles["type"] == ParticleTAGS.Exterior
-    lo = [(particles[f] < domain_manager.xmin[k]) & ext
-          for k, f in enumerate(particles.carray_named_groups["position"])]
-    hi = [(particles[f] > domain_manager.xmax[k]) & ext
-          for k, f in enumerate(particles.carray_named_groups["position"])]
-    return dim, lo, hi
+def _box(domain_manager):
+    lo = (ctypes.c_double * 3)(*(domain_manager.xmin + [0.0] * (3 - domain_manager.dim)))
+    hi = (ctypes.c_double * 3)(*(domain_manager.xmax + [0.0] * (3 - domain_manager.dim)))
+    return lo, hi
 
 
 class Reflective(BoundaryConditionBase):
     native_code = nat.BOUNDARY_REFLECTIVE
 
     def migrate_particles(self, particles, domain_manager):
-        real = particles["tag"] == ParticleTAGS.Real
-        inside = torch.ones_like(real)
-        for k, f in enumerate(particles.carray_named_groups["position"]):
-            x = particles[f]
-            inside &= (x >= domain_manager.xmin[k]) & (x <= domain_manager.xmax[k])
-        if bool((real & ~inside).any()):
+        """boundary.pyx:236-263: raise if a real particle left the box (the
+        check runs on the GPU and synchronises)."""
+        lib, h = nat.flux(), nat.handle()
+        c = cells_view(particles)
+        lo, hi = _box(domain_manager)
+        first = ctypes.c_int64(-1)
+        nat.check(lib.phd_first_outside(h, ctypes.byref(c), lo, hi, ctypes.byref(first)),
+                  "migrate_particles")
+        if first.value >= 0:
             raise RuntimeError("particle left domain in reflective boundary condition!!")
 
     def update_gradients(self, particles, gradients, domain_manager):
@@ -59,31 +55,23 @@ class Reflective(BoundaryConditionBase):
 
     def update_fields(self, particles, domain_manager):
         """Negate velocity and momentum of exterior ghosts on each axis where
-        they lie outside the box (boundary.pyx:336-348)."""
-        dim, lo, hi = _outside_masks(particles, domain_manager)
-        for k in range(dim):
-            v = particles[particles.carray_named_groups["velocity"][k]]
-            mv = particles[particles.carray_named_groups["momentum"][k]]
-            for mask in (lo[k], hi[k]):
-                v[mask] *= -1
-                mv[mask] *= -1
+        they lie outside the box (boundary.pyx:336-348), on the GPU."""
+        lib, h = nat.flux(), nat.handle()
+        c = cells_view(particles)
+        lo, hi = _box(domain_manager)
+        nat.check(lib.phd_reflect_fields(h, ctypes.byref(c), lo, hi), "update_fields")
 
 
 class Periodic(BoundaryConditionBase):
     native_code = nat.BOUNDARY_PERIODIC
 
     def migrate_particles(self, particles, domain_manager):
-        """Wrap real particles that left the box (boundary.pyx:523-538)."""
-        real = particles["tag"] == ParticleTAGS.Real
-        pos = [particles[f] for f in particles.carray_named_groups["position"]]
-        outside = torch.zeros_like(real)
-        for k, x in enumerate(pos):
-            outside |= (x < domain_manager.xmin[k]) | (x > domain_manager.xmax[k])
-        sel = real & outside
-        for k, x in enumerate(pos):
-            L = domain_manager.xmax[k] - domain_manager.xmin[k]
-            x[sel & (x <= domain_manager.xmin[k])] += L
-            x[sel & (x >= domain_manager.xmax[k])] -= L
+        """Wrap real particles that left the box (boundary.pyx:523-538), on
+        the GPU; both tests of an axis see the position before its wrap."""
+        lib, h = nat.flux(), nat.handle()
+        c = cells_view(particles)
+        lo, hi = _box(domain_manager)
+        nat.check(lib.phd_migrate_periodic(h, ctypes.byref(c), lo, hi), "migrate_particles")
 
     def update_gradients(self, particles, gradients, domain_manager):
         domain_manager._ghost_gradients(particles, gradients, nat.BOUNDARY_PERIODIC)
@@ -142,21 +130,23 @@ class DomainManager(object):
         self.boundary_condition.migrate_particles(particles, self)
 
     def move_generators(self, particles, dt):
-        """x += dt*w for real cells (domain_manager.pyx:566-590)."""
-        real = particles["tag"] == ParticleTAGS.Real
-        for f, w in zip(particles.carray_named_groups["position"],
-                        particles.carray_named_groups["w"]):
-            x = particles[f]
-            x[real] += dt * particles[w][real]
+        """x += dt*w for real cells (domain_manager.pyx:566-590), on the GPU."""
+        lib, h = nat.flux(), nat.handle()
+        c = cells_view(particles)
+        nat.check(lib.phd_move_generators(h, ctypes.byref(c), float(dt)), "move_generators")
 
     def update_ghost_fields(self, particles, fields, apply_boundary_condition=False):
-        """Copy fields from each ghost's image (domain_manager.pyx:759-774)."""
-        ghost = torch.nonzero(particles["tag"] == ParticleTAGS.Ghost).flatten()
-        if ghost.numel():
-            src = particles["map"][ghost].long()
-            for field in fields:
-                a = particles[field]
-                a[ghost] = a[src]
+        """Copy fields from each ghost's image (domain_manager.pyx:757-774),
+        on the GPU."""
+        fields = list(fields)
+        if fields and particles.get_carray_size():
+            lib, h = nat.flux(), nat.handle()
+            c = cells_view(particles)
+            ptrs = (ctypes.c_void_p * len(fields))(*[particles.pointer(f) for f in fields])
+            size = (ctypes.c_int32 * len(fields))(
+                *[particles.carrays[f].element_size() for f in fields])
+            nat.check(lib.phd_update_ghost_fields(h, ctypes.byref(c), ptrs, size, len(fields)),
+                      "update_ghost_fields")
         if apply_boundary_condition:
             self.boundary_condition.update_fields(particles, self)
 

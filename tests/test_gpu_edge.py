@@ -91,9 +91,17 @@ def test_empty_mesh_every_stage(gpu, dim):
                                       nat.SOLVER_HLLC, None, 0),
         "phd_ghost_refresh_eos":
             lambda: lib.phd_ghost_refresh_eos(h, p, cr, nat.BOUNDARY_REFLECTIVE, lo_, hi_, g),
+        "phd_move_generators": lambda: lib.phd_move_generators(h, cr, 0.01),
+        "phd_migrate_periodic": lambda: lib.phd_migrate_periodic(h, cr, lo_, hi_),
+        "phd_reflect_fields": lambda: lib.phd_reflect_fields(h, cr, lo_, hi_),
+        "phd_update_ghost_fields":
+            lambda: lib.phd_update_ghost_fields(h, cr, (ctypes.c_void_p * 1)(c.volume or 1),
+                                                (ctypes.c_int32 * 1)(8), 1),
     }
     failed = {name: rc for name, rc in ((n, call()) for n, call in calls.items()) if rc != 0}
     assert not failed, failed
+    first = ctypes.c_int64(-2)
+    assert lib.phd_first_outside(h, cr, lo_, hi_, ctypes.byref(first)) == 0 and first.value == -1
     dt = ctypes.c_double(-1.0)
     assert lib.phd_compute_time_step(h, cr, g, 0.3, 1, ctypes.byref(dt)) == nat.PHD_E_ARG
     torch.cuda.synchronize()
