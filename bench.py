@@ -11,14 +11,20 @@ A step is one GPU hydro step on that mesh, all inputs resident in HBM:
   MovingMeshMUSCLHancock.evolve_timestep, integrate.py:396-408), then the
   ghost conserved refresh + EOS epilogue (integrate.py:371-377) so the state
   stays physical from step to step.
-value = faces / (wall time per step).  The adjoint block times the forward
-and reverse of a checkpointed KH 256^2 rollout (50 steps, a checkpoint every
-10) and reports reverse/forward.  Per-kernel HIP-event times give the
-roofline line for the dominant kernel.  The CPU baseline is the oracle
-(serial C restatement of the reference loops) timed on one host core on one
-full core step of the same workload.
+value = faces / (wall time per step).  roofline is the core step (the four
+kernels of gradients .. update, HIP-event medians on the library's stream)
+against the algorithmic byte model B_core of that step; "kernels" carries
+each kernel's own line.  The adjoint block times the forward and reverse of
+a checkpointed KH 256^2 rollout (50 steps, a checkpoint every 10) and
+reports reverse/forward.  The ensemble block times 4096 independent 2D
+Riemann problems (1k-64k cells each) as one batched graph launch per step
+against the per-problem launches, over 100 steps.  The CPU baseline is the
+oracle (serial C restatement of the reference loops) on one host core, 20
+core steps of the same Sedov set-up at half the lattice side per axis, with
+one full-size step beside it; the OpenMP line runs 20 full-size steps.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128]
+                       [--no-cpu-baseline] [--no-adjoint] [--no-ensemble]
        python bench.py --workload ensemble [--problems 4096] [--steps 100]
                        [--schedule G,F,U,X]    (batched ensemble, its own line)
 """
@@ -45,6 +51,7 @@ KERNEL_NAMES = {"gradients": "k_grad_sell<3, 0, 256, 1>",
                 "update": "k_update_sell<3, 256, 1>", "ghost_gradients": "k_ghost_rec<3>",
                 "epilogue": "k_ghost_refresh_eos<3, 0>"}
 COPY_KERNEL = "k_pack_xp<3>"
+CORE_STAGES = ("gradients", "ghost_gradients", "face_flux", "update")
 
 
 def algorithmic_bytes(dim, nr, ng, nf):
@@ -60,8 +67,9 @@ def algorithmic_bytes(dim, nr, ng, nf):
                       + 4 * (na + 1) + 4 * 2 * nf + nr * 8 * ngrad)
     # ghost gradients: read image rows, write ghost rows, ghost list/map/type/x
     b["ghost_gradients"] = ng * (2 * 8 * ngrad + 4 + 4 + 1 + 8 * dim)
-    # fused face kernel: face geometry + pairs, cell prim/x/dcom/grad, flux out
-    b["face_flux"] = (nf * (8 * (3 * dim) + 8 + 8 * nfld)
+    # fused face kernel: face geometry (centroid, normal, velocity, area),
+    # pair_i/pair_j, cell prim/x/dcom/grad, flux out
+    b["face_flux"] = (nf * (8 * (3 * dim) + 8 + 4 * 2 + 8 * nfld)
                       + na * 8 * (nfld + 2 * dim + ngrad))
     # update: flux + area + pairs via the CSR, cons in/out, tags
     b["update"] = (nf * (8 * nfld + 8 + 8) + 4 * 2 * nf + 4 * (na + 1)
@@ -216,25 +224,57 @@ def stream_copy_gbs(nbytes=2 << 30, reps=10):
     return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
-def cpu_baseline(problem, dt):
-    """One full core step of the same workload through the oracle, one core."""
-    from phd import problems
+CPU_SAMPLES = 20
+
+
+def _timed_core_steps(a, dt, n, omp):
     from oracle import oracle
-    a = problems.host_arrays(problem)
-    oracle.lib()
+    oracle.lib(omp)
     times = []
-    while len(times) < 3 and sum(times) < 20.0:
+    for _ in range(n):
         t0 = time.perf_counter()
-        r = oracle.core_step(a, dt)
+        r = oracle.core_step(a, dt, omp=omp)
         times.append(time.perf_counter() - t0)
         if r["rc"] != 0:
             raise RuntimeError("oracle core step reported fault %d" % r["rc"])
+    return times
+
+
+def cpu_baseline(a_sample, dt_sample, n_sample, a_full, dt_full):
+    """The oracle's core step (gradients + limiter, ghost gradients, spatial +
+    temporal states, HLLC + deboost, update) on one host core.  The timed
+    sample is CPU_SAMPLES steps of the same Sedov set-up at n_sample^3 (a
+    128^3 step takes about 7 s on one core); one step of the full mesh is
+    timed beside it to show the per-face rate carries over."""
+    times = _timed_core_steps(a_sample, dt_sample, CPU_SAMPLES, omp=False)
     t = sorted(times)[len(times) // 2]
-    return {"value": a["nf"] / t, "unit": "face-updates/s", "cores": 1, "kind": "port",
-            "sample": "median of %d full core steps (gradients+limiter, ghost gradients, "
-                      "spatial+temporal states, HLLC+deboost, update) of the same Sedov mesh, "
-                      "%d faces: %s s each on one core of %s" %
-                      (len(times), a["nf"], ", ".join("%.2f" % x for x in times), cpu_model())}
+    full = _timed_core_steps(a_full, dt_full, 1, omp=False)[0] if a_full is not None else None
+    out = {"value": a_sample["nf"] / t, "unit": "face-updates/s", "cores": 1, "kind": "port",
+           "sample": "median of %d core steps of the Sedov set-up at %d^3 (%d faces), one core "
+                     "of %s: %.3f s per step (min %.3f, max %.3f)" %
+                     (len(times), n_sample, a_sample["nf"], cpu_model(), t, min(times),
+                      max(times))}
+    if full is not None:
+        out["full_size_check"] = {"faces": a_full["nf"], "seconds": round(full, 3),
+                                  "face_updates_per_s": a_full["nf"] / full}
+    return out
+
+
+def cpu_baseline_omp(a, dt, n):
+    """The same core step through the OpenMP build of the oracle (its
+    independent per-cell and per-face loops on all threads OpenMP is given;
+    the update stays serial, in face order), CPU_SAMPLES steps of the full
+    mesh: the all-cores CPU line."""
+    from oracle import oracle
+    threads = oracle.omp_threads()
+    times = _timed_core_steps(a, dt, CPU_SAMPLES, omp=True)
+    t = sorted(times)[len(times) // 2]
+    return {"value": a["nf"] / t, "unit": "face-updates/s", "cores": threads, "kind": "port",
+            "sample": "median of %d core steps of the %d^3 Sedov mesh (%d faces) through the "
+                      "OpenMP build of the oracle, %d threads on %s (%d logical CPUs): %.3f s "
+                      "per step (min %.3f, max %.3f)" %
+                      (len(times), n, a["nf"], threads, cpu_model(), os.cpu_count() or 0, t,
+                       min(times), max(times))}
 
 
 def adjoint_timing(n=256, n_steps=50, every=10, reps=3):
@@ -367,29 +407,6 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline_omp(problem, dt):
-    """The same core step through the OpenMP build of the oracle (its
-    independent per-cell and per-face loops on all threads OpenMP is given;
-    the update stays serial, in face order): the all-cores CPU line."""
-    from phd import problems
-    from oracle import oracle
-    a = problems.host_arrays(problem)
-    threads = oracle.omp_threads()
-    times = []
-    while len(times) < 5 and sum(times) < 20.0:
-        t0 = time.perf_counter()
-        r = oracle.core_step(a, dt, omp=True)
-        times.append(time.perf_counter() - t0)
-        if r["rc"] != 0:
-            raise RuntimeError("oracle core step reported fault %d" % r["rc"])
-    t = sorted(times)[len(times) // 2]
-    return {"value": a["nf"] / t, "unit": "face-updates/s", "cores": threads, "kind": "port",
-            "sample": "median of %d full core steps of the same Sedov mesh through the OpenMP "
-                      "build of the oracle, %d threads on %s (%d logical CPUs): %s s each" %
-                      (len(times), threads, cpu_model(), os.cpu_count() or 0,
-                       ", ".join("%.2f" % x for x in times))}
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -398,6 +415,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the batched-ensemble block of the default run")
     ap.add_argument("--workload", choices=("sedov", "ensemble"), default="sedov",
                     help="ensemble: only the batched-ensemble measurement, as its own line")
     ap.add_argument("--problems", type=int, default=4096)
@@ -521,22 +540,32 @@ def main():
     core_ms = sum(kern_ms[n] for n in ("gradients", "ghost_gradients", "face_flux", "update"))
 
     B = algorithmic_bytes(prob.dim, nr, ng, nf)
-    dom = max(("gradients", "face_flux", "update"), key=lambda n: kern_ms[n])
-    achieved = B[dom] / (kern_ms[dom] * 1e-3) / 1e9
-    traffic = measured_traffic(args.n, dom)
+    # headline roofline: the core step (gradients, ghost gradients, face,
+    # update) against the algorithmic byte model B_core of the whole step
+    core_traffic = [measured_traffic(args.n, k) for k in CORE_STAGES]
+    achieved = B["core_model"] / (core_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None if traffic is None else round(traffic),
-                "kernel": dom, "algorithmic_bytes": B[dom]}
-
-    vi = measured_valu(args.n, dom)
-    valu_line = None
-    if vi is not None:
-        floor_ms = vi / VALU_WAVE_INSTR_PER_S * 1e3
-        valu_line = {"kernel": dom, "wave_instr_per_launch": vi, "floor_ms": round(floor_ms, 4),
-                     "frac": round(floor_ms / kern_ms[dom], 4),
-                     "note": "FP64 VALU issue utilisation: instructions x 4 cycles over "
-                             "256 CUs x 4 SIMDs at 2.4 GHz, against the kernel's time"}
+                "traffic": None if None in core_traffic else round(sum(core_traffic)),
+                "kernel": "core step: " + ", ".join(KERNEL_NAMES[k] for k in CORE_STAGES),
+                "algorithmic_bytes": B["core_model"], "time_ms": round(core_ms, 4),
+                "bytes_per_face": round(B["core_model"] / nf, 2)}
+    # per-kernel lines: each kernel's own minimum bytes, time and measured
+    # traffic / FP64 VALU issue utilisation from the committed counter record
+    kernels = {}
+    for k in ("gradients", "ghost_gradients", "face_flux", "update", "epilogue"):
+        line = {"kernel": KERNEL_NAMES[k], "ms": round(kern_ms[k], 4),
+                "algorithmic_bytes": B[k],
+                "achieved_gbs": round(B[k] / (kern_ms[k] * 1e-3) / 1e9, 1),
+                "frac_hbm": round(B[k] / (kern_ms[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        tr = measured_traffic(args.n, k)
+        line["traffic"] = None if tr is None else round(tr)
+        vi = measured_valu(args.n, k)
+        if vi is not None:
+            floor_ms = vi / VALU_WAVE_INSTR_PER_S * 1e3
+            line["valu_floor_ms"] = round(floor_ms, 4)
+            line["valu_frac"] = round(floor_ms / kern_ms[k], 4)
+        kernels[k] = line
     result = {
         "metric": "Voronoi face-flux updates/s FP64 (3D Sedov 2M cells) + %HBM roofline; "
                   "adjoint/fwd",
@@ -559,20 +588,38 @@ def main():
         "kernel_ms": {k: round(v, 4) for k, v in kern_ms.items()},
         "core_step_ms": round(core_ms, 4),
         "core_face_updates_per_s": nf / (core_ms * 1e-3),
-        "core_model_frac_hbm": round(B["core_model"] / (core_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,
-        "valu": valu_line,
+        "kernels": kernels,
         "stream_copy_gbs": round(stream_copy_gbs(), 1),
         "cpu_baseline": None,
         "cpu_baseline_omp": None,
         "adjoint": None,
+        "ensemble": None,
     }
+    # host copies for the CPU lines, then free the device problem
+    a_full = problems.host_arrays(prob) if not args.no_cpu_baseline else None
+    del L, step, prob, plan
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     if not args.no_adjoint:
         result["adjoint"] = adjoint_timing()
         result["adjoint_over_fwd"] = result["adjoint"]["ratio"]
+        torch.cuda.empty_cache()
+    if not args.no_ensemble:
+        def log(m):
+            print(m, file=sys.stderr, flush=True)
+        result["ensemble"] = ensemble_timing(args.problems, steps=100, log=log)
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(prob, dt)
-        result["cpu_baseline_omp"] = cpu_baseline_omp(prob, dt)
+        n_s = max(8, args.n // 2)
+        ps = problems.sedov_3d(n=n_s, seed=0)
+        st = problems.FrozenStep(ps)
+        st.prologue()
+        dt_s = st.time_step()
+        a_s = problems.host_arrays(ps)
+        del st, ps
+        result["cpu_baseline"] = cpu_baseline(a_s, dt_s, n_s, a_full, dt)
+        result["cpu_baseline_omp"] = cpu_baseline_omp(a_full, dt, args.n)
     print(json.dumps(result), flush=True)
 
 
