@@ -45,6 +45,7 @@ extern "C" {
 #define PHD_E_ARG      (-1)   /* bad argument (dim, sizes, NULL pointer) */
 #define PHD_E_HIP      (-2)   /* a HIP runtime call failed */
 #define PHD_E_MODE     (-3)   /* unsupported solver / limiter / mode */
+#define PHD_E_SIZE     (-4)   /* input larger than a fixed-size scratch allows */
 /* device-detected numeric faults (phd_get_error) */
 #define PHD_F_NEIGHBORS 1     /* "ERROR: incorrect neighbors!"  reconstruction.pyx:650-652 */
 #define PHD_F_MASS      2     /* "Mass less than zero in flux update"  mesh.pyx:546-548 */
@@ -155,6 +156,20 @@ int phd_primitive_from_conservative(phd_handle* h, const phd_cells* c, double ga
  * writes cfl * min over real cells to *dt_out (host). */
 int phd_compute_time_step(phd_handle* h, const phd_cells* c, double gamma,
                           double cfl, int boost, double* dt_out);
+
+/* Allocation-free variant of phd_compute_time_step.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (harnesses that
+ * forbid hipMalloc, serving loops).  Needs no handle and allocates nothing:
+ * its per-block partial minima live in a fixed device array of 512 KiB per
+ * GPU (resolved with hipGetSymbolAddress on the first call on each device
+ * and held until the process ends).  Runs on `stream` (a hipStream_t of the
+ * current device; NULL = default stream) and synchronises it.  Same bits as
+ * phd_compute_time_step.  At most phd_noalloc_max_cells() cells (16,777,216):
+ * larger inputs return PHD_E_SIZE.  Calls that overlap on one GPU share its
+ * array and race. */
+int phd_compute_time_step_noalloc(const phd_cells* c, double gamma, double cfl,
+                                  int boost, void* stream, double* dt_out);
+int phd_noalloc_max_cells(void);
 
 /* ---- mesh motion (mesh/mesh.pyx) ---------------------------------------- */
 /* Mesh.assign_generator_velocities, mesh.pyx:363-426 (all cells) */

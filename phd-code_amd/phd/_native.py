@@ -24,7 +24,7 @@ c_dbl = ctypes.c_double
 
 # constants mirrored from include/phd_flux.h
 PHD_ABI_VERSION = 1
-PHD_E_ARG, PHD_E_HIP, PHD_E_MODE = -1, -2, -3
+PHD_E_ARG, PHD_E_HIP, PHD_E_MODE, PHD_E_SIZE = -1, -2, -3, -4
 PHD_F_NEIGHBORS, PHD_F_MASS, PHD_F_ENERGY, PHD_F_EXACT = 1, 2, 3, 4
 SOLVER_HLL, SOLVER_HLLC, SOLVER_EXACT = 0, 1, 2
 LIMITER_AREPO, LIMITER_TESS = 0, 1
@@ -124,6 +124,9 @@ _FLUX_SIGS = {
     "phd_first_outside": ([c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_i64)], c_i32),
     "phd_reflect_fields": ([c_vp, c_vp, c_vp, c_vp], c_i32),
     "phd_update_ghost_fields": ([c_vp, c_vp, c_vp, c_vp, c_i32], c_i32),
+    "phd_compute_time_step_noalloc": ([c_vp, c_dbl, c_dbl, c_i32, c_vp, ctypes.POINTER(c_dbl)],
+                                      c_i32),
+    "phd_noalloc_max_cells": ([], c_i32),
 }
 
 # symbols added by later parts of the library register here too

@@ -1499,6 +1499,7 @@ const char* phd_error_string(int code)
     case PHD_E_ARG: return "invalid argument";
     case PHD_E_HIP: return "HIP runtime error";
     case PHD_E_MODE: return "unsupported mode";
+    case PHD_E_SIZE: return "input exceeds the fixed scratch size of an allocation-free entry point";
     case PHD_F_NEIGHBORS: return "ERROR: incorrect neighbors!";
     case PHD_F_MASS: return "Mass less than zero in flux update";
     case PHD_F_ENERGY: return "Energy less than zero in flux update";
