@@ -26,7 +26,7 @@ one full-size step beside it; the OpenMP line runs 20 full-size steps.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128]
                        [--no-cpu-baseline] [--no-adjoint] [--no-ensemble]
        python bench.py --workload ensemble [--problems 4096] [--steps 100]
-                       [--schedule G,F,U,X]    (batched ensemble, its own line)
+                       [--schedule G,F,U,X] [--batched-only]   (batched ensemble, its own line)
 """
 import argparse
 import json
@@ -316,7 +316,8 @@ def adjoint_timing(n=256, n_steps=50, every=10, reps=3):
             "ratio": round(bwd / fwd, 3)}
 
 
-def ensemble_timing(n_problems=4096, steps=100, seed=0, log=None, schedule=None):
+def ensemble_timing(n_problems=4096, steps=100, seed=0, log=None, schedule=None,
+                    batched_only=False):
     """Batched ensemble of independent 2D Riemann problems (cells per problem
     log-uniform in [1k, 64k]): autotuned schedule, one graph launch per step.
     Both forms run the same `steps` steps from the same initial state and are
@@ -325,7 +326,9 @@ def ensemble_timing(n_problems=4096, steps=100, seed=0, log=None, schedule=None)
     phd_ghost_refresh_eos on its own plan, with the batched run's per-problem
     time steps, so no host synchronisation).  The two final states are
     compared bit for bit.  schedule: a fixed (grad, face, update block, xcd)
-    schedule instead of the autotuner's search."""
+    schedule instead of the autotuner's search.  batched_only: time the
+    batched run alone (no per-problem form; for profiling the batched
+    kernels)."""
     import torch
     from phd import ensemble as E
     log = log or (lambda m: None)
@@ -356,6 +359,15 @@ def ensemble_timing(n_problems=4096, steps=100, seed=0, log=None, schedule=None)
     _, k, hist, faults = ens.read(history=True)
     ens.clear_errors()            # frozen problems' failed updates land there too
     assert k == steps
+    if batched_only:
+        ms_b = wall_b * 1e3 / (steps - 1)
+        out = {"workload": "%d independent 2D Riemann problems, batched run only" % n_problems,
+               "cells": ens.n_cells, "faces": ens.n_faces, "steps": steps,
+               "batched_ms_per_step": round(ms_b, 3),
+               "batched_face_updates_per_s": ens.n_faces / (ms_b * 1e-3),
+               "schedule": dict(zip(("grad_block", "face_block", "update_block", "xcd"), best))}
+        ens.close()
+        return out
     after_b = ens.state()
     # per-problem form over the same steps
     t2 = time.time()
@@ -420,6 +432,8 @@ def main():
     ap.add_argument("--workload", choices=("sedov", "ensemble"), default="sedov",
                     help="ensemble: only the batched-ensemble measurement, as its own line")
     ap.add_argument("--problems", type=int, default=4096)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="ensemble: time the batched run only (no per-problem form)")
     ap.add_argument("--schedule", default=None, metavar="G,F,U,X",
                     help="ensemble: a fixed schedule (threads per block of the gradient, "
                          "face and update kernels, XCD remap 0/1) instead of the autotuner")
@@ -451,7 +465,8 @@ def main():
         def log(m):
             print(m, file=sys.stderr, flush=True)
         sched = [int(v) for v in args.schedule.split(",")] if args.schedule else None
-        r = ensemble_timing(args.problems, steps=args.steps, log=log, schedule=sched)
+        r = ensemble_timing(args.problems, steps=args.steps, log=log, schedule=sched,
+                            batched_only=args.batched_only)
         r.update({"metric": "batched ensemble face-updates/s FP64 (one graph launch per step) "
                             "and speed-up over per-problem launches",
                   "value": r["batched_face_updates_per_s"], "unit": "face-updates/s",
