@@ -298,6 +298,58 @@ int phd_update_ghost_fields(phd_handle* h, const phd_cells* c,
                             void* const* fields, const int32_t* elem_size,
                             int32_t n_fields);
 
+/* ---- mesh geometry from Delaunay topology --------------------------------
+ * The device half of Tess2d::extract_geometry (tess.cpp:238-398) and
+ * Tess3d::extract_geometry (tess3.cpp:334-562), called by Mesh.build_geometry
+ * (mesh.pyx:255-323) after the host's topology-only build
+ * (phdm_build_topology, phd_mesh.h).  The topology is the host's triangulation
+ * in the builder's layout: faces in owner order (pair_i real, pair_j > pair_i
+ * or a ghost), ghosts n_real .. n_real + n_ghost - 1 with their image's real
+ * index and image code, each face's ring (the generators whose bisectors
+ * bound it, in polygon order) and the per-cell face lists.  All arrays are
+ * device arrays. */
+typedef struct phd_topology {
+    int32_t        dim;        /* 2 or 3 */
+    int32_t        boundary;   /* 0 reflective, 1 periodic */
+    int32_t        n_real, n_ghost, n_faces;
+    double         lo[3], hi[3];
+    const int32_t* ghost_map;  /* n_ghost: real index of each ghost's image */
+    const int8_t*  ghost_code; /* n_ghost: image code */
+    const int32_t* pair_i;     /* n_faces */
+    const int32_t* pair_j;
+    const int32_t* ring_off;   /* n_faces + 1 */
+    const int32_t* ring_k;     /* ring entries: real index ... */
+    const int8_t*  ring_code;  /* ... and image code */
+    const int32_t* nbr_off;    /* n_real + n_ghost + 1 */
+    const int32_t* nbr_face;   /* face ids, ascending within each cell */
+} phd_topology;
+
+/* Outputs (and the generator positions) of phd_extract_geometry. */
+typedef struct phd_geometry {
+    double* x[3];       /* n_real + n_ghost: real rows read, ghost rows written */
+    double* volume;     /* n_real + n_ghost */
+    double* dcom[3];    /* n_real + n_ghost, centroid - generator */
+    double* area;       /* n_faces */
+    double* com[3];
+    double* normal[3];
+} phd_geometry;
+
+/* Bytes of the workspace phd_extract_geometry needs: 2 (1 + dim) doubles
+ * per face (each side's volume term and centroid moment). */
+size_t phd_geometry_workspace_bytes(int dim, int64_t n_faces);
+/* Ghost positions (the reference's image formulas, boundary.pyx:103-104,
+ * 378-380), then face area, centroid and normal and cell volume and dcom,
+ * with the reference's formulas (Voronoi vertices are circumcentres of the
+ * ring's Delaunay simplices; 3D faces by a fan about the vertex mean and the
+ * zero-area rule of tess3.cpp:494-502; volumes by triangles / tetrahedra on
+ * the faces; ghost volume and dcom copied from the image).  Each face is
+ * computed once, in its owner generator's frame, and each cell sums its
+ * faces in ascending face id.  Synchronising.  A topology entry out of range
+ * returns PHD_E_ARG and *bad = the first such face, or -2 - c for cell (or
+ * ghost) c. */
+int phd_extract_geometry(phd_handle* h, const phd_topology* t, const phd_geometry* g,
+                         void* workspace, size_t workspace_bytes, int64_t* bad);
+
 /* ---- batched ensembles ---------------------------------------------------
  * No counterpart in the reference, which runs one problem per process.
  * Many independent problems are laid out as one union mesh: problem p owns

@@ -56,6 +56,36 @@ int phdm_fill(const phdm_mesh* m, double* const* ghost_x, int32_t* ghost_map,
               double* volume, double* const* dcom, double* area,
               double* const* com, double* const* normal, int32_t* pair_i,
               int32_t* pair_j, int32_t* nbr_off, int32_t* nbr_face);
+
+/* Topology-only build (the host half of GPU geometry extraction): the same
+ * faces, pairs, ghosts, ghost positions and per-cell face lists as
+ * phdm_build, and no face or cell geometry -- phdm_fill returns -2 if asked
+ * for volume, dcom, area, com or normal of such a mesh.  The geometry is then
+ * computed on the device by phd_extract_geometry (phd_flux.h) from the
+ * rings below.  Replaces the CGAL triangulation that
+ * Tess2d/Tess3d::extract_geometry (tess.cpp:238-398, tess3.cpp:334-562)
+ * walk. */
+phdm_mesh* phdm_build_topology(int dim, int64_t n_real, const double* const* x,
+                               const double* lo, const double* hi, int boundary,
+                               int n_threads, int* status);
+/* Number of ring entries (sum of ring lengths over faces); any mesh. */
+int phdm_ring_count(const phdm_mesh* m, int64_t* n_ring);
+/* The Delaunay star of each face (i, j) = (pair_i, pair_j), any mesh:
+ *   ring_off[f] .. ring_off[f+1]   (n_faces + 1 offsets)
+ *   ring_k[r], ring_code[r]        the generator: real index and image code
+ *   ghost_code[g]                  image code of each ghost (g < n_ghost)
+ * Image code 0 is the generator itself; reflective 1 + 2*axis + side
+ * (side 1 = upper wall), the mirror x - 2 (x - wall) (boundary.pyx:103-104);
+ * periodic 1 + sum_a (s_a + 1) 3^a for the shift s in {-1, 0, 1}^dim, the
+ * copy x + s translate (boundary.pyx:368-380).  ghost_x is that image of
+ * ghost_map's generator.
+ * In 2D a ring has 2 entries and vertex t (t = 0, 1) of the face is the
+ * circumcentre of generators i, j and ring[t]; in 3D vertex t of the
+ * face polygon (t < m, the ring length) is the circumcentre of i, j, ring[t]
+ * and ring[(t + 1) mod m] -- the Delaunay tetrahedra around edge (i, j),
+ * whose duals CGAL's cell circulator visits (tess3.cpp:423-440). */
+int phdm_fill_topology(const phdm_mesh* m, int8_t* ghost_code, int32_t* ring_off,
+                       int32_t* ring_k, int8_t* ring_code);
 void phdm_free(phdm_mesh* m);
 
 #ifdef __cplusplus

@@ -294,3 +294,32 @@ def rollout_dual(a, prim0, dprim0, dts, boost=True, gizmo=True):
     if rc != 0:
         raise RuntimeError("oracle rollout fault %d" % rc)
     return out, dout
+
+
+def extract_geometry(dim, x, topo, lo, hi, boundary, omp=False):
+    """Mesh geometry from Delaunay topology (orc_extract_geometry).
+
+    x: the real generators' coordinates (dim arrays of n); topo: a dict with
+    ghost_map, ghost_code, pair_i, pair_j, ring_off, ring_k, ring_code,
+    nbr_off, nbr_face (phd.mesh.build_voronoi's keys).  Returns a dict with
+    x (reals then ghost images), volume, dcom (n + n_ghost), area, com,
+    normal (n_faces)."""
+    n = x[0].size
+    ng, nf = topo["ghost_map"].size, topo["pair_i"].size
+    X = [np.concatenate([np.asarray(x[k], dtype=np.float64), np.zeros(ng)]) for k in range(dim)]
+    out = {"x": X, "volume": np.zeros(n + ng), "dcom": [np.zeros(n + ng) for _ in range(dim)],
+           "area": np.zeros(nf), "com": [np.zeros(nf) for _ in range(dim)],
+           "normal": [np.zeros(nf) for _ in range(dim)]}
+    ws = np.zeros(max(1, nf * 2 * (dim + 1)))
+    lo_ = np.array(list(lo) + [0.0] * (3 - dim), dtype=np.float64)
+    hi_ = np.array(list(hi) + [0.0] * (3 - dim), dtype=np.float64)
+    i32 = {k: np.ascontiguousarray(topo[k], dtype=np.int32)
+           for k in ("ghost_map", "pair_i", "pair_j", "ring_off", "ring_k", "nbr_off", "nbr_face")}
+    i8 = {k: np.ascontiguousarray(topo[k], dtype=np.int8) for k in ("ghost_code", "ring_code")}
+    lib(omp).orc_extract_geometry(
+        dim, int(boundary), _p(lo_), _p(hi_), n, ng, nf, _pp(X), _p(i32["ghost_map"]),
+        _p(i8["ghost_code"]), _p(i32["pair_i"]), _p(i32["pair_j"]), _p(i32["ring_off"]),
+        _p(i32["ring_k"]), _p(i8["ring_code"]), _p(i32["nbr_off"]), _p(i32["nbr_face"]),
+        _p(out["volume"]), _pp(out["dcom"]), _p(out["area"]), _pp(out["com"]),
+        _pp(out["normal"]), _p(ws))
+    return out

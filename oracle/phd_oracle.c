@@ -974,3 +974,274 @@ int orc_core_step(int dim, int n, int nf, double gamma, double dt, int solver,
     return orc_update_from_fluxes(dim, nf, dt, tag, area, pair_i, pair_j, flux,
                                   cons, bad);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Geometry of a Voronoi mesh from its Delaunay topology: the face set
+ * (pair_i, pair_j), ghost images (ghost_map, ghost_code) and the ring of
+ * each face -- the generators whose bisectors bound it, in polygon order
+ * (include/phd_mesh.h, phdm_fill_topology).  Follows
+ * Tess2d::extract_geometry (tess.cpp:238-398) and
+ * Tess3d::extract_geometry (tess3.cpp:334-562):
+ *   - Voronoi vertices are circumcentres of Delaunay triangles/tetrahedra
+ *     (CGAL's tess.dual(cc), tess3.cpp:435-437);
+ *   - 2D: area = |p2 - p1|, com = midpoint, volume += area h / 4, centroid
+ *     += (area h / 4)(2 f / 3 + x / 3) (tess.cpp:313-355), dcom = c / vol -
+ *     x (tess.cpp:389-393), normal = (x_j - x_i) / h;
+ *   - 3D: fan about the vertex mean c (tess3.cpp:452-483), the zero-area
+ *     rule (tess3.cpp:494-502), com = sum(c3 a3) / area1, area = area1 / 2,
+ *     normal = (x_j - x_i) * (1 / |x_j - x_i|) (tess3.cpp:504-517),
+ *     volume and centroid from the tetrahedra (face centroid, v1, v2,
+ *     generator) (tess3.cpp:537-559);
+ *   - ghost positions: periodic x + (index - 1) translate
+ *     (boundary.pyx:378-380), reflective x - 2 (x - edge)
+ *     (boundary.pyx:103-104, 125-126); ghost volume and dcom are copies of
+ *     the image's (mesh.pyx:141-142, 323).
+ * Two choices the reference leaves to CGAL are fixed here: every face is
+ * computed once, in the frame of its owner generator x_o (o = pair_i), and
+ * feeds both cells -- cell o with the apex at 0, cell j with the apex at
+ * x_j - x_o; and a cell's volume and centroid sum its faces' contributions
+ * in ascending face id (the order of its face list), the centroid relative
+ * to its own generator.  The reference's zero-area test compares
+ * sqrt(area1) with the distance of the unnormalised centroid sum from the
+ * generator, in the owner frame |sum c3 a3|; a face with area1 == 0 is
+ * zero as well (the reference reaches the same via its absolute frame).  A
+ * zero face keeps its place in the face list with area 0 and com = x_o + c
+ * and adds nothing to the volumes.
+ * A circumcentre of 0, a, b (2D) / 0, a, b, c (3D) solves p.q = |q|^2 / 2:
+ *   2D: p = (|a|^2 (b_y, -b_x) + |b|^2 (-a_y, a_x)) / (2 (a_x b_y - a_y b_x))
+ *   3D: p = (|a|^2 (b x c) + |b|^2 (c x a) + |c|^2 (a x b)) / (2 a.(b x c))
+ * The per-face contributions live in ws: 2 sides x (1 + dim) doubles per
+ * face (volume term, then the centroid moments). */
+
+/* image `code` of a generator coordinate xk on axis a */
+static double orc_image(int boundary, const double* lo, const double* hi,
+                        double xk, int a, int code)
+{
+    if (code == 0) return xk;
+    if (boundary == 1) {
+        int idx = code - 1, b;
+        for (b = 0; b < a; b++) idx /= 3;
+        return xk + (double)(idx % 3 - 1)*(hi[a] - lo[a]);
+    }
+    if ((code - 1) >> 1 != a) return xk;
+    {
+        const double edge = ((code - 1) & 1) ? hi[a] : lo[a];
+        return xk - 2*(xk - edge);
+    }
+}
+
+/* ring member r relative to xo */
+static void orc_ring_pos(int dim, int boundary, const double* lo, const double* hi,
+                         double* const* x, int32_t k, int code, const double* xo,
+                         double* p)
+{
+    int a;
+    for (a = 0; a < dim; a++)
+        p[a] = orc_image(boundary, lo, hi, x[a][k], a, code) - xo[a];
+}
+
+static void orc_circ2(const double* a, const double* b, double* p)
+{
+    const double aa = a[0]*a[0] + a[1]*a[1], bb = b[0]*b[0] + b[1]*b[1];
+    const double d = a[0]*b[1] - a[1]*b[0];
+    const double r = 1.0/(d + d);
+    p[0] = (aa*b[1] - bb*a[1])*r;
+    p[1] = (bb*a[0] - aa*b[0])*r;
+}
+
+static void orc_cross(const double* u, const double* v, double* w)
+{
+    w[0] = u[1]*v[2] - u[2]*v[1];
+    w[1] = u[2]*v[0] - u[0]*v[2];
+    w[2] = u[0]*v[1] - u[1]*v[0];
+}
+
+static void orc_circ3(const double* a, const double* b, const double* c, double* p)
+{
+    double bc[3], ca[3], ab[3];
+    int k;
+    const double aa = a[0]*a[0] + a[1]*a[1] + a[2]*a[2];
+    const double bb = b[0]*b[0] + b[1]*b[1] + b[2]*b[2];
+    const double cc = c[0]*c[0] + c[1]*c[1] + c[2]*c[2];
+    orc_cross(b, c, bc);
+    orc_cross(c, a, ca);
+    orc_cross(a, b, ab);
+    {
+        const double det = a[0]*bc[0] + a[1]*bc[1] + a[2]*bc[2];
+        const double r = 1.0/(det + det);
+        for (k = 0; k < 3; k++) p[k] = (aa*bc[k] + bb*ca[k] + cc*ab[k])*r;
+    }
+}
+
+/* vertex t of 3D face f (ring r0 .. r0+m), relative to xo */
+static void orc_vertex3(int boundary, const double* lo, const double* hi,
+                        double* const* x, const int32_t* ring_k,
+                        const int8_t* ring_code, int r0, int m, int t,
+                        const double* xo, const double* a, double* w)
+{
+    double p[3], q[3];
+    const int u = r0 + t, v = r0 + (t + 1 == m ? 0 : t + 1);
+    orc_ring_pos(3, boundary, lo, hi, x, ring_k[u], ring_code[u], xo, p);
+    orc_ring_pos(3, boundary, lo, hi, x, ring_k[v], ring_code[v], xo, q);
+    orc_circ3(a, p, q, w);
+}
+
+static void orc_face_geometry(int dim, int boundary, const double* lo,
+                              const double* hi, double* const* x,
+                              const int32_t* pair_i, const int32_t* pair_j,
+                              const int32_t* ring_off, const int32_t* ring_k,
+                              const int8_t* ring_code, int f, double* area,
+                              double* const* com, double* const* normal,
+                              double* ws)
+{
+    const int o = pair_i[f], j = pair_j[f];
+    const int r0 = ring_off[f], m = ring_off[f + 1] - ring_off[f];
+    const int W = 1 + dim;
+    double xo[3], a[3];
+    int k, t;
+    for (k = 0; k < dim; k++) {
+        xo[k] = x[k][o];
+        a[k] = x[k][j] - xo[k];
+    }
+    if (dim == 2) {
+        double p[2], w0[2], w1[2], fm[2];
+        double xe, ye, ar, h, wt;
+        orc_ring_pos(2, boundary, lo, hi, x, ring_k[r0], ring_code[r0], xo, p);
+        orc_circ2(a, p, w0);
+        orc_ring_pos(2, boundary, lo, hi, x, ring_k[r0 + 1], ring_code[r0 + 1], xo, p);
+        orc_circ2(a, p, w1);
+        h = sqrt(a[0]*a[0] + a[1]*a[1]);
+        xe = w1[0] - w0[0];
+        ye = w1[1] - w0[1];
+        ar = sqrt(xe*xe + ye*ye);
+        fm[0] = 0.5*(w0[0] + w1[0]);
+        fm[1] = 0.5*(w0[1] + w1[1]);
+        wt = 0.25*ar*h;
+        area[f] = ar;
+        for (k = 0; k < 2; k++) {
+            normal[k][f] = a[k]/h;
+            com[k][f] = xo[k] + fm[k];
+            ws[(size_t)f*2*W + 1 + k] = wt*(2.0*fm[k]/3.0);
+            ws[(size_t)f*2*W + W + 1 + k] = wt*(2.0*(fm[k] - a[k])/3.0);
+        }
+        ws[(size_t)f*2*W] = wt;
+        ws[(size_t)f*2*W + W] = wt;
+        return;
+    }
+    {
+        const double third = 1.0/3.0, fourth = 0.25;
+        double c[3] = {0.0, 0.0, 0.0}, s[3] = {0.0, 0.0, 0.0};
+        double w[3], v1[3], v2[3], n3[3], fc[3], cvo[3], cvj[3];
+        double area1 = 0.0, vo = 0.0, vj = 0.0;
+        double mo[3] = {0.0, 0.0, 0.0}, mj[3] = {0.0, 0.0, 0.0};
+        const double hn = 1.0/sqrt(a[0]*a[0] + a[1]*a[1] + a[2]*a[2]);
+        for (k = 0; k < 3; k++) normal[k][f] = a[k]*hn;
+        /* vertex mean */
+        for (t = 0; t < m; t++) {
+            orc_vertex3(boundary, lo, hi, x, ring_k, ring_code, r0, m, t, xo, a, w);
+            for (k = 0; k < 3; k++) c[k] += w[k];
+        }
+        for (k = 0; k < 3; k++) c[k] *= 1.0/m;
+        /* fan about c: area and unnormalised centroid */
+        orc_vertex3(boundary, lo, hi, x, ring_k, ring_code, r0, m, m - 1, xo, a, w);
+        for (k = 0; k < 3; k++) v1[k] = w[k] - c[k];
+        for (t = 0; t < m; t++) {
+            double a3;
+            orc_vertex3(boundary, lo, hi, x, ring_k, ring_code, r0, m, t, xo, a, w);
+            for (k = 0; k < 3; k++) v2[k] = w[k] - c[k];
+            orc_cross(v1, v2, n3);
+            a3 = sqrt(n3[0]*n3[0] + n3[1]*n3[1] + n3[2]*n3[2]);
+            for (k = 0; k < 3; k++) s[k] += (c[k] + (v1[k] + v2[k])*third)*a3;
+            area1 += a3;
+            for (k = 0; k < 3; k++) v1[k] = v2[k];
+        }
+        {
+            const double l1 = sqrt(area1);
+            const double l2 = sqrt(s[0]*s[0] + s[1]*s[1] + s[2]*s[2]);
+            if (area1 == 0.0 || l1 < 1.0e-10*l2) {
+                area[f] = 0.0;
+                for (k = 0; k < 3; k++) com[k][f] = xo[k] + c[k];
+                for (k = 0; k < 2*W; k++) ws[(size_t)f*2*W + k] = 0.0;
+                return;
+            }
+        }
+        for (k = 0; k < 3; k++) fc[k] = s[k]*(1.0/area1);
+        area[f] = 0.5*area1;
+        for (k = 0; k < 3; k++) {
+            com[k][f] = xo[k] + fc[k];
+            cvo[k] = 0.0 - fc[k];
+            cvj[k] = a[k] - fc[k];
+        }
+        /* tetrahedra (face centroid, v1, v2, apex) for both cells */
+        orc_vertex3(boundary, lo, hi, x, ring_k, ring_code, r0, m, m - 1, xo, a, w);
+        for (k = 0; k < 3; k++) v1[k] = w[k] - fc[k];
+        for (t = 0; t < m; t++) {
+            double cr[3], v4;
+            orc_vertex3(boundary, lo, hi, x, ring_k, ring_code, r0, m, t, xo, a, w);
+            for (k = 0; k < 3; k++) v2[k] = w[k] - fc[k];
+            orc_cross(v1, v2, cr);
+            v4 = fabs(cr[0]*cvo[0] + cr[1]*cvo[1] + cr[2]*cvo[2]);
+            vo += v4;
+            for (k = 0; k < 3; k++) mo[k] += (fc[k] + (v1[k] + v2[k] + cvo[k])*fourth)*v4;
+            v4 = fabs(cr[0]*cvj[0] + cr[1]*cvj[1] + cr[2]*cvj[2]);
+            vj += v4;
+            for (k = 0; k < 3; k++)
+                mj[k] += ((fc[k] + (v1[k] + v2[k] + cvj[k])*fourth) - a[k])*v4;
+            for (k = 0; k < 3; k++) v1[k] = v2[k];
+        }
+        ws[(size_t)f*2*W] = vo;
+        ws[(size_t)f*2*W + W] = vj;
+        for (k = 0; k < 3; k++) {
+            ws[(size_t)f*2*W + 1 + k] = mo[k];
+            ws[(size_t)f*2*W + W + 1 + k] = mj[k];
+        }
+    }
+}
+
+void orc_extract_geometry(int dim, int boundary, const double* lo, const double* hi,
+                          int n, int ng, int nf, double* const* x,
+                          const int32_t* ghost_map, const int8_t* ghost_code,
+                          const int32_t* pair_i, const int32_t* pair_j,
+                          const int32_t* ring_off, const int32_t* ring_k,
+                          const int8_t* ring_code, const int32_t* nbr_off,
+                          const int32_t* nbr_face, double* volume,
+                          double* const* dcom, double* area, double* const* com,
+                          double* const* normal, double* ws)
+{
+    const int W = 1 + dim;
+    int g, f, i;
+    ORC_PFOR()
+    for (g = 0; g < ng; g++) {
+        int a;
+        for (a = 0; a < dim; a++)
+            x[a][n + g] = orc_image(boundary, lo, hi, x[a][ghost_map[g]], a, ghost_code[g]);
+    }
+    ORC_PFOR()
+    for (f = 0; f < nf; f++)
+        orc_face_geometry(dim, boundary, lo, hi, x, pair_i, pair_j, ring_off,
+                          ring_k, ring_code, f, area, com, normal, ws);
+    ORC_PFOR()
+    for (i = 0; i < n; i++) {
+        double v = 0.0, mm[3] = {0.0, 0.0, 0.0};
+        int e, k;
+        for (e = nbr_off[i]; e < nbr_off[i + 1]; e++) {
+            const int fe = nbr_face[e];
+            const double* q = ws + (size_t)fe*2*W + (pair_i[fe] == i ? 0 : W);
+            v += q[0];
+            for (k = 0; k < dim; k++) mm[k] += q[1 + k];
+        }
+        if (dim == 2) {
+            volume[i] = v;
+            for (k = 0; k < 2; k++) dcom[k][i] = mm[k]/v;
+        } else {
+            for (k = 0; k < 3; k++) dcom[k][i] = mm[k]*(1.0/v);
+            volume[i] = v*(1.0/6.0);
+        }
+    }
+    ORC_PFOR()
+    for (g = 0; g < ng; g++) {
+        int k;
+        volume[n + g] = volume[ghost_map[g]];
+        for (k = 0; k < dim; k++) dcom[k][n + g] = dcom[k][ghost_map[g]];
+    }
+}

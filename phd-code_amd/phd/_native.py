@@ -66,6 +66,18 @@ class Faces(ctypes.Structure):
                 ("nbr_off", c_vp), ("nbr_face", c_vp)]
 
 
+class Topology(ctypes.Structure):
+    _fields_ = [("dim", c_i32), ("boundary", c_i32), ("n_real", c_i32), ("n_ghost", c_i32),
+                ("n_faces", c_i32), ("lo", c_dbl * 3), ("hi", c_dbl * 3), ("ghost_map", c_vp),
+                ("ghost_code", c_vp), ("pair_i", c_vp), ("pair_j", c_vp), ("ring_off", c_vp),
+                ("ring_k", c_vp), ("ring_code", c_vp), ("nbr_off", c_vp), ("nbr_face", c_vp)]
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [("x", c_vp * 3), ("volume", c_vp), ("dcom", c_vp * 3), ("area", c_vp),
+                ("com", c_vp * 3), ("normal", c_vp * 3)]
+
+
 PtrArray5 = c_vp * 5
 DblArray3 = c_dbl * 3
 
@@ -124,6 +136,9 @@ _FLUX_SIGS = {
     "phd_first_outside": ([c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_i64)], c_i32),
     "phd_reflect_fields": ([c_vp, c_vp, c_vp, c_vp], c_i32),
     "phd_update_ghost_fields": ([c_vp, c_vp, c_vp, c_vp, c_i32], c_i32),
+    "phd_geometry_workspace_bytes": ([c_i32, c_i64], ctypes.c_size_t),
+    "phd_extract_geometry": ([c_vp, c_vp, c_vp, c_vp, ctypes.c_size_t, ctypes.POINTER(c_i64)],
+                             c_i32),
     "phd_compute_time_step_noalloc": ([c_vp, c_dbl, c_dbl, c_i32, c_vp, ctypes.POINTER(c_dbl)],
                                       c_i32),
     "phd_noalloc_max_cells": ([], c_i32),
@@ -137,6 +152,10 @@ _MESH_SIGS = {
     "phdm_counts": ([c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),
                      ctypes.POINTER(c_i64)], c_i32),
     "phdm_fill": ([c_vp] + [c_vp] * 11, c_i32),
+    "phdm_build_topology": ([c_i32, c_i64, c_vp, c_vp, c_vp, c_i32, c_i32, ctypes.POINTER(c_i32)],
+                            c_vp),
+    "phdm_ring_count": ([c_vp, ctypes.POINTER(c_i64)], c_i32),
+    "phdm_fill_topology": ([c_vp] + [c_vp] * 4, c_i32),
     "phdm_free": ([c_vp], None),
 }
 

@@ -15,7 +15,7 @@ REPO = os.path.dirname(os.path.dirname(HERE))
 INCLUDE = os.path.join(REPO, "include")
 CSRC = os.path.join(HERE, "csrc")
 
-HIP_SOURCES = ["phd_flux.hip", "adjoint.hip", "domain.hip", "noalloc.hip"]
+HIP_SOURCES = ["phd_flux.hip", "adjoint.hip", "domain.hip", "noalloc.hip", "geometry.hip"]
 # -ffp-contract=off: no FMA contraction, so BITWISE kernels perform the same
 # IEEE operations as the reference; packed-fp32 instructions are disabled.
 HIPCC_FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared",

@@ -40,6 +40,7 @@ struct FaceRec {
     double area;
     double com[3];    // absolute
     double q[3];      // neighbor position - generator (absolute offset)
+    int32_t r0, rn;   // the face's ring: Work::ring[r0 .. r0 + rn)
 };
 
 struct CellOut {
@@ -259,9 +260,13 @@ double rmax2_2(const Poly2& P)
 
 struct phdm_mesh {
     int dim;
+    bool has_geometry;
     int64_t n_real, n_ghost, n_faces;
     std::vector<double> gx[3];
     std::vector<int32_t> gmap;
+    std::vector<int8_t> gcode;
+    std::vector<int32_t> ring_off, ring_k;
+    std::vector<int8_t> ring_code;
     std::vector<double> volume, dcom[3];
     std::vector<double> area, com[3], normal[3];
     std::vector<int32_t> pi, pj;
@@ -358,12 +363,31 @@ struct Work {
     std::vector<double> s;
     std::vector<int> remap, ea, eb, ev, nxt;
     std::vector<Cand> cands;
+    std::vector<NKey> ring;                            // rings of the emitted faces
+    std::vector<std::pair<int64_t, int>> edges;        // directed edge -> face
 };
 
+// The face across edge (u -> v) of a face of P: the face holding (v -> u).
+// W.edges holds every directed edge of P, sorted.  -1 if there is none.
+int twin_face(const Work& W, int u, int v)
+{
+    const int64_t key = ((int64_t)v << 32) | (uint32_t)u;
+    auto it = std::lower_bound(W.edges.begin(), W.edges.end(), std::make_pair(key, -1));
+    if (it == W.edges.end() || it->first != key) return -1;
+    return it->second;
+}
+
 // Build real cell i; appends its emitted faces (neighbor real index > i, or
-// any image) to `faces`.  Returns false if the cell could not be closed.
+// any image) to `faces` and their rings to W.ring.  The ring of a face is the
+// Delaunay star of the edge (i, j): the keys of the generators whose bisector
+// planes bound the face, in the order of the face's edges.  In 2D it is the
+// two neighbours meeting the face at its end points (vertex t is the
+// circumcentre of i, j, ring[t]); in 3D ring[t] is the face across the
+// face's edge t, so vertex t is the circumcentre of i, j, ring[t],
+// ring[t + 1 mod m].  geometry = false skips the cell's volume and centroid.
+// Returns false if the cell could not be closed.
 bool build_cell(const Grid& g, const double* const* x, int64_t i, bool periodic, Work& W,
-                std::vector<FaceRec>& faces, CellOut& co)
+                std::vector<FaceRec>& faces, CellOut& co, bool geometry)
 {
     const int dim = g.dim;
     double xi[3] = {x[0][i], x[1][i], dim == 3 ? x[2][i] : 0.0};
@@ -462,15 +486,21 @@ bool build_cell(const Grid& g, const double* const* x, int64_t i, bool periodic,
             double area = std::sqrt(xe * xe + ye * ye);
             if (area <= 1e-14 * h) continue;
             double fx = 0.5 * (x1 + x2), fy = 0.5 * (y1 + y2);
-            vol += 0.25 * area * h;
-            cen[0] += 0.25 * area * h * (2.0 * fx / 3.0);
-            cen[1] += 0.25 * area * h * (2.0 * fy / 3.0);
+            if (geometry) {
+                vol += 0.25 * area * h;
+                cen[0] += 0.25 * area * h * (2.0 * fx / 3.0);
+                cen[1] += 0.25 * area * h * (2.0 * fy / 3.0);
+            }
             const NKey& t = P.etag[a];
             if (t.code != 0 || t.k > i) {
                 FaceRec fr;
                 fr.k = t.k; fr.code = t.code; fr.area = area;
                 fr.com[0] = xi[0] + fx; fr.com[1] = xi[1] + fy; fr.com[2] = 0;
                 fr.q[0] = q[0]; fr.q[1] = q[1]; fr.q[2] = 0;
+                // end points: a on edge a-1, a+1 on edge a+1
+                fr.r0 = (int32_t)W.ring.size(); fr.rn = 2;
+                W.ring.push_back(P.etag[(a + nv - 1) % nv]);
+                W.ring.push_back(P.etag[b]);
                 faces.push_back(fr);
             }
         }
@@ -480,6 +510,13 @@ bool build_cell(const Grid& g, const double* const* x, int64_t i, bool periodic,
         co.dcom[2] = 0;
     } else {
         const Poly3& P = use_a ? W.p3a : W.p3b;
+        W.edges.clear();
+        for (int f = 0; f < P.nf(); f++)
+            for (int t = 0; t < P.fcnt[f]; t++) {
+                const int u = P.fv[P.fstart[f] + t], v = P.fv[P.fstart[f] + (t + 1) % P.fcnt[f]];
+                W.edges.push_back(std::make_pair(((int64_t)u << 32) | (uint32_t)v, f));
+            }
+        std::sort(W.edges.begin(), W.edges.end());
         for (int f = 0; f < P.nf(); f++) {
             const int st = P.fstart[f], nvtx = P.fcnt[f];
             double c[3] = {0, 0, 0};
@@ -504,6 +541,20 @@ bool build_cell(const Grid& g, const double* const* x, int64_t i, bool periodic,
             double qn = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
             if (area1 <= 1e-20 * qn * qn) continue;   // degenerate facet
             for (int a = 0; a < 3; a++) fc[a] *= 1.0 / area1;
+            const NKey& tg = P.ftag[f];
+            if (tg.code != 0 || tg.k > i) {
+                FaceRec fr;
+                fr.k = tg.k; fr.code = tg.code; fr.area = 0.5 * area1;
+                for (int a = 0; a < 3; a++) { fr.com[a] = xi[a] + fc[a]; fr.q[a] = q[a]; }
+                fr.r0 = (int32_t)W.ring.size(); fr.rn = nvtx;
+                for (int t = 0; t < nvtx; t++) {
+                    const int g_ = twin_face(W, P.fv[st + t], P.fv[st + (t + 1) % nvtx]);
+                    if (g_ < 0) return false;   // open polyhedron
+                    W.ring.push_back(P.ftag[g_]);
+                }
+                faces.push_back(fr);
+            }
+            if (!geometry) continue;
             // tetrahedra (facet centroid, v1, v2, generator) -- tess3.cpp:537-549
             const double fourth = 0.25;
             double cv[3] = {-fc[0], -fc[1], -fc[2]};
@@ -518,13 +569,6 @@ bool build_cell(const Grid& g, const double* const* x, int64_t i, bool periodic,
                 for (int a = 0; a < 3; a++) cen[a] += (fc[a] + (v1[a] + v2[a] + cv[a]) * fourth) * v4;
                 for (int a = 0; a < 3; a++) v1[a] = v2[a];
             }
-            const NKey& tg = P.ftag[f];
-            if (tg.code != 0 || tg.k > i) {
-                FaceRec fr;
-                fr.k = tg.k; fr.code = tg.code; fr.area = 0.5 * area1;
-                for (int a = 0; a < 3; a++) { fr.com[a] = xi[a] + fc[a]; fr.q[a] = q[a]; }
-                faces.push_back(fr);
-            }
         }
         for (int a = 0; a < 3; a++) co.dcom[a] = cen[a] / vol;
         co.volume = vol / 6.0;
@@ -532,12 +576,26 @@ bool build_cell(const Grid& g, const double* const* x, int64_t i, bool periodic,
     return true;
 }
 
-} // namespace
+// Position of the image `code` of generator k along axis a, as the
+// reference's ghost creation places it: a periodic shift by (index - 1) *
+// translate (boundary.pyx:378-380), a reflective mirror x - 2 (x - edge)
+// (boundary.pyx:103-104, 125-126).
+double image_coord(const Grid& g, bool periodic, double xk, int a, int32_t code)
+{
+    if (code == 0) return xk;
+    if (periodic) {
+        int idx = code - 1;
+        for (int b = 0; b < a; b++) idx /= 3;
+        return xk + (double)(idx % 3 - 1) * g.L[a];
+    }
+    const int axis = (code - 1) >> 1, side = (code - 1) & 1;
+    if (axis != a) return xk;
+    const double edge = side ? g.hi[a] : g.lo[a];
+    return xk - 2 * (xk - edge);
+}
 
-extern "C" {
-
-phdm_mesh* phdm_build(int dim, int64_t n, const double* const* x, const double* lo,
-                      const double* hi, int boundary, int n_threads, int* status)
+phdm_mesh* build_mesh(int dim, int64_t n, const double* const* x, const double* lo,
+                      const double* hi, int boundary, int n_threads, int* status, bool geometry)
 {
     if (status) *status = 0;
     if ((dim != 2 && dim != 3) || n < 1 || !x || !lo || !hi ||
@@ -591,6 +649,7 @@ phdm_mesh* phdm_build(int dim, int64_t n, const double* const* x, const double* 
     const int64_t CH = 2048;
     const int64_t nch = (n + CH - 1) / CH;
     std::vector<std::vector<FaceRec>> chunk_faces(nch);
+    std::vector<std::vector<NKey>> chunk_ring(nch);
     std::vector<std::vector<int32_t>> chunk_cnt(nch);
     std::vector<CellOut> cells(n);
     int fail = 0;
@@ -602,10 +661,12 @@ phdm_mesh* phdm_build(int dim, int64_t n, const double* const* x, const double* 
         for (int64_t ch = 0; ch < nch; ch++) {
             int64_t b = ch * CH, e = std::min(n, b + CH);
             std::vector<FaceRec>& out = chunk_faces[ch];
+            std::vector<NKey>& rout = chunk_ring[ch];
             chunk_cnt[ch].resize(e - b);
             for (int64_t i = b; i < e; i++) {
                 tmp.clear();
-                if (!build_cell(g, x, i, periodic, W, tmp, cells[i])) {
+                W.ring.clear();
+                if (!build_cell(g, x, i, periodic, W, tmp, cells[i], geometry)) {
 #pragma omp atomic write
                     fail = 1;
                 }
@@ -618,6 +679,11 @@ phdm_mesh* phdm_build(int dim, int64_t n, const double* const* x, const double* 
                     return A.code < B.code;
                 });
                 chunk_cnt[ch][i - b] = (int32_t)tmp.size();
+                for (FaceRec& fr : tmp) {
+                    const int32_t r0 = (int32_t)rout.size();
+                    rout.insert(rout.end(), W.ring.begin() + fr.r0, W.ring.begin() + fr.r0 + fr.rn);
+                    fr.r0 = r0;
+                }
                 out.insert(out.end(), tmp.begin(), tmp.end());
             }
         }
@@ -629,19 +695,27 @@ phdm_mesh* phdm_build(int dim, int64_t n, const double* const* x, const double* 
 
     phdm_mesh* M = new phdm_mesh();
     M->dim = dim;
+    M->has_geometry = geometry;
     M->n_real = n;
-    int64_t nf = 0;
+    int64_t nf = 0, nring = 0;
     for (auto& v : chunk_faces) nf += (int64_t)v.size();
-    if (nf > INT32_MAX / 2) {
+    for (auto& v : chunk_ring) nring += (int64_t)v.size();
+    if (nf > INT32_MAX / 2 || nring > INT32_MAX) {
         delete M;
         if (status) *status = -4;
         return nullptr;
     }
     M->n_faces = nf;
-    M->area.resize(nf);
     M->pi.resize(nf);
     M->pj.resize(nf);
-    for (int a = 0; a < dim; a++) { M->com[a].resize(nf); M->normal[a].resize(nf); }
+    M->ring_off.resize(nf + 1);
+    M->ring_k.resize(nring);
+    M->ring_code.resize(nring);
+    M->ring_off[0] = 0;
+    if (geometry) {
+        M->area.resize(nf);
+        for (int a = 0; a < dim; a++) { M->com[a].resize(nf); M->normal[a].resize(nf); }
+    }
     // ghost numbering in order of first appearance along the face list
     std::unordered_map<uint64_t, int32_t> gid;
     gid.reserve(1024);
@@ -663,40 +737,52 @@ phdm_mesh* phdm_build(int dim, int64_t n, const double* const* x, const double* 
                         int32_t id = (int32_t)M->gmap.size();
                         gid.emplace(key, id);
                         M->gmap.push_back((int32_t)fr.k);
+                        M->gcode.push_back((int8_t)fr.code);
                         for (int a = 0; a < dim; a++)
-                            M->gx[a].push_back(x[a][i] + fr.q[a]);
+                            M->gx[a].push_back(image_coord(g, periodic, x[a][fr.k], a, fr.code));
                         j = n + id;
                     } else {
                         j = n + it->second;
                     }
                 }
-                M->area[fcount] = fr.area;
                 M->pi[fcount] = (int32_t)i;
                 M->pj[fcount] = (int32_t)j;
-                double qn = 0;
-                for (int a = 0; a < dim; a++) qn += fr.q[a] * fr.q[a];
-                qn = std::sqrt(qn);
-                for (int a = 0; a < dim; a++) {
-                    M->com[a][fcount] = fr.com[a];
-                    M->normal[a][fcount] = fr.q[a] / qn;
+                const int32_t r0 = M->ring_off[fcount];
+                for (int32_t r = 0; r < fr.rn; r++) {
+                    M->ring_k[r0 + r] = (int32_t)chunk_ring[ch][fr.r0 + r].k;
+                    M->ring_code[r0 + r] = (int8_t)chunk_ring[ch][fr.r0 + r].code;
+                }
+                M->ring_off[fcount + 1] = r0 + fr.rn;
+                if (geometry) {
+                    M->area[fcount] = fr.area;
+                    double qn = 0;
+                    for (int a = 0; a < dim; a++) qn += fr.q[a] * fr.q[a];
+                    qn = std::sqrt(qn);
+                    for (int a = 0; a < dim; a++) {
+                        M->com[a][fcount] = fr.com[a];
+                        M->normal[a][fcount] = fr.q[a] / qn;
+                    }
                 }
                 fcount++;
             }
         }
         std::vector<FaceRec>().swap(chunk_faces[ch]);
+        std::vector<NKey>().swap(chunk_ring[ch]);
     }
     M->n_ghost = (int64_t)M->gmap.size();
     const int64_t na = n + M->n_ghost;
-    M->volume.resize(na);
-    for (int a = 0; a < dim; a++) M->dcom[a].resize(na);
-    for (int64_t i = 0; i < n; i++) {
-        M->volume[i] = cells[i].volume;
-        for (int a = 0; a < dim; a++) M->dcom[a][i] = cells[i].dcom[a];
-    }
-    for (int64_t gg = 0; gg < M->n_ghost; gg++) {
-        int64_t k = M->gmap[gg];
-        M->volume[n + gg] = cells[k].volume;
-        for (int a = 0; a < dim; a++) M->dcom[a][n + gg] = cells[k].dcom[a];
+    if (geometry) {
+        M->volume.resize(na);
+        for (int a = 0; a < dim; a++) M->dcom[a].resize(na);
+        for (int64_t i = 0; i < n; i++) {
+            M->volume[i] = cells[i].volume;
+            for (int a = 0; a < dim; a++) M->dcom[a][i] = cells[i].dcom[a];
+        }
+        for (int64_t gg = 0; gg < M->n_ghost; gg++) {
+            int64_t k = M->gmap[gg];
+            M->volume[n + gg] = cells[k].volume;
+            for (int a = 0; a < dim; a++) M->dcom[a][n + gg] = cells[k].dcom[a];
+        }
     }
     // per-cell face lists, ascending face id (tess.cpp:379-382)
     M->nbr_off.assign(na + 1, 0);
@@ -711,6 +797,22 @@ phdm_mesh* phdm_build(int dim, int64_t n, const double* const* x, const double* 
         }
     }
     return M;
+}
+
+} // namespace
+
+extern "C" {
+
+phdm_mesh* phdm_build(int dim, int64_t n, const double* const* x, const double* lo,
+                      const double* hi, int boundary, int n_threads, int* status)
+{
+    return build_mesh(dim, n, x, lo, hi, boundary, n_threads, status, true);
+}
+
+phdm_mesh* phdm_build_topology(int dim, int64_t n, const double* const* x, const double* lo,
+                               const double* hi, int boundary, int n_threads, int* status)
+{
+    return build_mesh(dim, n, x, lo, hi, boundary, n_threads, status, false);
 }
 
 int phdm_counts(const phdm_mesh* m, int64_t* n_ghost, int64_t* n_faces, int64_t* n_list)
@@ -729,6 +831,12 @@ int phdm_fill(const phdm_mesh* m, double* const* ghost_x, int32_t* ghost_map, do
     if (!m) return -1;
     const int dim = m->dim;
     const int64_t ng = m->n_ghost, na = m->n_real + m->n_ghost, nf = m->n_faces;
+    if (!m->has_geometry) {
+        bool asked = volume || area;
+        for (int a = 0; a < dim; a++)
+            asked = asked || (dcom && dcom[a]) || (com && com[a]) || (normal && normal[a]);
+        if (asked) return -2;   // a topology-only mesh holds no geometry
+    }
     for (int a = 0; a < dim; a++) {
         if (ghost_x && ghost_x[a] && ng) memcpy(ghost_x[a], m->gx[a].data(), ng * sizeof(double));
         if (dcom && dcom[a]) memcpy(dcom[a], m->dcom[a].data(), na * sizeof(double));
@@ -742,6 +850,25 @@ int phdm_fill(const phdm_mesh* m, double* const* ghost_x, int32_t* ghost_map, do
     if (pair_j && nf) memcpy(pair_j, m->pj.data(), nf * sizeof(int32_t));
     if (nbr_off) memcpy(nbr_off, m->nbr_off.data(), (na + 1) * sizeof(int32_t));
     if (nbr_face && nf) memcpy(nbr_face, m->nbr_face.data(), 2 * nf * sizeof(int32_t));
+    return 0;
+}
+
+int phdm_ring_count(const phdm_mesh* m, int64_t* n_ring)
+{
+    if (!m || !n_ring) return -1;
+    *n_ring = (int64_t)m->ring_k.size();
+    return 0;
+}
+
+int phdm_fill_topology(const phdm_mesh* m, int8_t* ghost_code, int32_t* ring_off,
+                       int32_t* ring_k, int8_t* ring_code)
+{
+    if (!m) return -1;
+    const int64_t ng = m->n_ghost, nf = m->n_faces, nr = (int64_t)m->ring_k.size();
+    if (ghost_code && ng) memcpy(ghost_code, m->gcode.data(), ng);
+    if (ring_off) memcpy(ring_off, m->ring_off.data(), (nf + 1) * sizeof(int32_t));
+    if (ring_k && nr) memcpy(ring_k, m->ring_k.data(), nr * sizeof(int32_t));
+    if (ring_code && nr) memcpy(ring_code, m->ring_code.data(), nr);
     return 0;
 }
 

@@ -1,11 +1,17 @@
-"""Voronoi mesh: host geometry build + GPU velocities and conserved update.
+"""Voronoi mesh: host topology + device geometry, GPU velocities and
+conserved update.
 
 Mesh keeps the reference's constructor, field registration and face schema
 (mesh/mesh.pyx:16-166).  build_geometry replaces the CGAL tessellation and
-ghost creation (mesh.pyx:168-323) by the host builder libphd_mesh.so and
-uploads the face arrays and per-cell face lists to the device.  The three hot
-methods (assign_generator_velocities, assign_face_velocities,
-update_from_fluxes; mesh.pyx:363-551) are HIP kernels.
+ghost creation (mesh.pyx:168-323): the host builder libphd_mesh.so finds the
+Delaunay topology (faces, ghost images, each face's ring of bounding
+generators, per-cell face lists), which is uploaded, and
+phd_extract_geometry computes ghost positions, face area, centroid and
+normal and cell volume and centroid on the device (tess.cpp:238-398,
+tess3.cpp:334-562).  device_geometry=False keeps the host builder's own
+geometry instead.  The three hot methods (assign_generator_velocities,
+assign_face_velocities, update_from_fluxes; mesh.pyx:363-551) are HIP
+kernels.
 """
 import ctypes
 
@@ -69,7 +75,7 @@ def face_lists_from_pairs(pair_i, pair_j, n_cells):
 
 class Mesh(object):
     def __init__(self, regularize=True, relax_iterations=0, eta=0.25, num_neighbors=128,
-                 max_iterations=20):
+                 max_iterations=20, device_geometry=True):
         self.particle_fields_registered = False
         self.max_iterations = max_iterations
         self.relax_iterations = relax_iterations
@@ -82,6 +88,8 @@ class Mesh(object):
         self._plan = None
         self._plan_key = None
         self.version = 0
+        self.device_geometry = bool(device_geometry)
+        self.topology = None
 
     def register_fields(self, particles):
         dim = len(particles.carray_named_groups["position"])
@@ -130,38 +138,53 @@ class Mesh(object):
         xs = [np.ascontiguousarray(particles[f].cpu().numpy(), dtype=np.float64)
               for f in particles.carray_named_groups["position"]]
         boundary = domain_manager.boundary_condition.native_code
-        geo = build_voronoi(dim, xs, domain_manager.xmin, domain_manager.xmax, boundary,
-                            self.n_threads)
+        lo, hi = domain_manager.xmin, domain_manager.xmax
+        geo = build_voronoi(dim, xs, lo, hi, boundary, self.n_threads,
+                            geometry=not self.device_geometry)
         ng = geo["ghost_map"].size
         dev = particles.device
         if ng:
             gmap = torch.as_tensor(geo["ghost_map"], device=dev, dtype=torch.long)
             ghosts = particles.extract_items(gmap)
-            for k, f in enumerate(particles.carray_named_groups["position"]):
-                ghosts[f][:] = torch.as_tensor(geo["ghost_x"][k], device=dev)
+            if not self.device_geometry:
+                for k, f in enumerate(particles.carray_named_groups["position"]):
+                    ghosts[f][:] = torch.as_tensor(geo["ghost_x"][k], device=dev)
             ghosts["tag"][:] = ParticleTAGS.Ghost
             ghosts["type"][:] = ParticleTAGS.Exterior
             if "map" in ghosts.carrays:
                 ghosts["map"][:] = gmap.to(torch.int32)
             particles.append_container(ghosts)
-        particles["volume"][:] = torch.as_tensor(geo["volume"], device=dev)
-        for k, f in enumerate(particles.carray_named_groups["dcom"]):
-            particles[f][:] = torch.as_tensor(geo["dcom"][k], device=dev)
-        self.set_faces(geo)
+        if not self.device_geometry:
+            particles["volume"][:] = torch.as_tensor(geo["volume"], device=dev)
+            for k, f in enumerate(particles.carray_named_groups["dcom"]):
+                particles[f][:] = torch.as_tensor(geo["dcom"][k], device=dev)
+            self.set_faces(geo)
+            return
+        self.topology = upload_topology(geo, dev)
+        self.faces.resize(geo["pair_i"].size)
+        extract_geometry(self.topology, dim, lo, hi, boundary,
+                         [particles[f] for f in particles.carray_named_groups["position"]],
+                         particles["volume"],
+                         [particles[f] for f in particles.carray_named_groups["dcom"]],
+                         self.faces)
+        self.set_faces(geo, device_geometry=True)
 
-    def set_faces(self, geo):
-        """Upload face arrays and face lists (host numpy dict)."""
-        nf = geo["area"].size
+    def set_faces(self, geo, device_geometry=False):
+        """Upload face arrays and face lists (host numpy dict); with
+        device_geometry the area, com and normal arrays are already on the
+        device and only the pairs and face lists are taken from geo."""
+        nf = geo["pair_i"].size
         fc = self.faces
         fc.resize(nf)
         dev = fc.device
-        fc["area"][:] = torch.as_tensor(geo["area"], device=dev)
         fc["pair-i"][:] = torch.as_tensor(geo["pair_i"], device=dev)
         fc["pair-j"][:] = torch.as_tensor(geo["pair_j"], device=dev)
-        for k, f in enumerate(fc.carray_named_groups["com"]):
-            fc[f][:] = torch.as_tensor(geo["com"][k], device=dev)
-        for k, f in enumerate(fc.carray_named_groups["normal"]):
-            fc[f][:] = torch.as_tensor(geo["normal"][k], device=dev)
+        if not device_geometry:
+            fc["area"][:] = torch.as_tensor(geo["area"], device=dev)
+            for k, f in enumerate(fc.carray_named_groups["com"]):
+                fc[f][:] = torch.as_tensor(geo["com"][k], device=dev)
+            for k, f in enumerate(fc.carray_named_groups["normal"]):
+                fc[f][:] = torch.as_tensor(geo["normal"][k], device=dev)
         for f in fc.carray_named_groups["velocity"]:
             fc[f][:] = 0.0
         self.neighbors = FaceLists(torch.as_tensor(geo["nbr_off"], device=dev),
@@ -218,8 +241,67 @@ class Mesh(object):
             nat.raise_device_fault(h)
 
 
-def build_voronoi(dim, xs, lo, hi, boundary, n_threads=0):
-    """Run the host builder; returns a dict of numpy arrays."""
+TOPOLOGY_KEYS = ("ghost_map", "ghost_code", "pair_i", "pair_j", "ring_off", "ring_k",
+                 "ring_code", "nbr_off", "nbr_face")
+
+
+def upload_topology(geo, device):
+    """Device copies of the topology arrays of a build_voronoi result."""
+    t = {k: torch.as_tensor(geo[k], device=device) for k in TOPOLOGY_KEYS}
+    t["n_real"] = int(geo["n_real"])
+    return t
+
+
+def extract_geometry(topo, dim, lo, hi, boundary, x, volume, dcom, faces=None, area=None,
+                     com=None, normal=None):
+    """phd_extract_geometry on device tensors: topo from upload_topology;
+    x, dcom: dim tensors of n_real + n_ghost (ghost rows of x are written);
+    volume likewise; the face outputs either a face container (faces) or
+    the area, com and normal tensors."""
+    if faces is not None:
+        area = faces["area"]
+        com = [faces[f] for f in faces.carray_named_groups["com"]]
+        normal = [faces[f] for f in faces.carray_named_groups["normal"]]
+    lib, h = nat.flux(), nat.handle()
+    nf = int(topo["pair_i"].numel())
+    t = nat.Topology()
+    t.dim, t.boundary = dim, int(boundary)
+    t.n_real, t.n_ghost, t.n_faces = topo["n_real"], int(topo["ghost_map"].numel()), nf
+    for a in range(dim):
+        t.lo[a], t.hi[a] = float(lo[a]), float(hi[a])
+    for k in TOPOLOGY_KEYS:
+        v = topo[k]
+        setattr(t, k, v.data_ptr() if v.numel() else None)
+
+    def ptr(v):
+        assert v.dtype == torch.float64 and v.is_contiguous()
+        return v.data_ptr()
+
+    g = nat.Geometry()
+    for a in range(dim):
+        g.x[a], g.dcom[a] = ptr(x[a]), ptr(dcom[a])
+        if nf:
+            g.com[a], g.normal[a] = ptr(com[a]), ptr(normal[a])
+    g.volume = ptr(volume)
+    g.area = ptr(area) if nf else None
+    nbytes = int(lib.phd_geometry_workspace_bytes(dim, nf))
+    ws = torch.empty(max(1, nbytes // 8), dtype=torch.float64, device=volume.device)
+    bad = ctypes.c_int64(-1)
+    rc = lib.phd_extract_geometry(h, ctypes.byref(t), ctypes.byref(g), ws.data_ptr(), nbytes,
+                                  ctypes.byref(bad))
+    if rc == nat.PHD_E_ARG and bad.value != -1:
+        raise RuntimeError("phd_extract_geometry: invalid topology entry (%s %d)" % (
+            ("face", bad.value) if bad.value >= 0 else ("cell", -2 - bad.value)))
+    nat.check(rc, "extract_geometry")
+
+
+def build_voronoi(dim, xs, lo, hi, boundary, n_threads=0, geometry=True):
+    """Run the host builder; returns a dict of numpy arrays: the topology
+    (ghost_x, ghost_map, ghost_code, pair_i, pair_j, nbr_off, nbr_face and
+    the face rings ring_off, ring_k, ring_code) and, with geometry=True, the
+    host-computed geometry (volume, dcom, area, com, normal).  geometry=False
+    runs the topology-only build, whose geometry phd_extract_geometry
+    computes on the device."""
     lib = nat.mesh_lib()
     n = xs[0].size
     xs = [np.ascontiguousarray(x, dtype=np.float64) for x in xs]
@@ -227,9 +309,9 @@ def build_voronoi(dim, xs, lo, hi, boundary, n_threads=0):
     lo_ = (ctypes.c_double * 3)(*[float(v) for v in lo], *([0.0] * (3 - dim)))
     hi_ = (ctypes.c_double * 3)(*[float(v) for v in hi], *([0.0] * (3 - dim)))
     st = ctypes.c_int32(0)
-    m = lib.phdm_build(dim, n, ctypes.cast(xp, ctypes.c_void_p), ctypes.cast(lo_, ctypes.c_void_p),
-                       ctypes.cast(hi_, ctypes.c_void_p), int(boundary), int(n_threads),
-                       ctypes.byref(st))
+    build = lib.phdm_build if geometry else lib.phdm_build_topology
+    m = build(dim, n, ctypes.cast(xp, ctypes.c_void_p), ctypes.cast(lo_, ctypes.c_void_p),
+              ctypes.cast(hi_, ctypes.c_void_p), int(boundary), int(n_threads), ctypes.byref(st))
     if not m:
         msg = {-1: "bad arguments", -2: "particle left domain (generator outside the box)",
                -3: "Mesh failed to converged!", -4: "too many faces"}.get(st.value, "error")
@@ -238,34 +320,45 @@ def build_voronoi(dim, xs, lo, hi, boundary, n_threads=0):
         ng, nf, nl = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         lib.phdm_counts(m, ctypes.byref(ng), ctypes.byref(nf), ctypes.byref(nl))
         ng, nf, nl = ng.value, nf.value, nl.value
+        nr_ = ctypes.c_int64()
+        lib.phdm_ring_count(m, ctypes.byref(nr_))
         na = n + ng
         out = {
             "ghost_x": [np.zeros(ng) for _ in range(dim)],
             "ghost_map": np.zeros(ng, dtype=np.int32),
-            "volume": np.zeros(na),
-            "dcom": [np.zeros(na) for _ in range(dim)],
-            "area": np.zeros(nf),
-            "com": [np.zeros(nf) for _ in range(dim)],
-            "normal": [np.zeros(nf) for _ in range(dim)],
+            "ghost_code": np.zeros(ng, dtype=np.int8),
             "pair_i": np.zeros(nf, dtype=np.int32),
             "pair_j": np.zeros(nf, dtype=np.int32),
             "nbr_off": np.zeros(na + 1, dtype=np.int32),
             "nbr_face": np.zeros(nl, dtype=np.int32),
+            "ring_off": np.zeros(nf + 1, dtype=np.int32),
+            "ring_k": np.zeros(nr_.value, dtype=np.int32),
+            "ring_code": np.zeros(nr_.value, dtype=np.int8),
         }
+        if geometry:
+            out.update({"volume": np.zeros(na), "dcom": [np.zeros(na) for _ in range(dim)],
+                        "area": np.zeros(nf), "com": [np.zeros(nf) for _ in range(dim)],
+                        "normal": [np.zeros(nf) for _ in range(dim)]})
+        keep = []
 
-        def parr(lst):
-            a = (ctypes.c_void_p * 3)(*[x.ctypes.data for x in lst], *([0] * (3 - dim)))
-            out.setdefault("_keep", []).append(a)
+        def parr(key):
+            if key not in out:
+                return None
+            a = (ctypes.c_void_p * 3)(*[x.ctypes.data for x in out[key]], *([0] * (3 - dim)))
+            keep.append(a)
             return ctypes.cast(a, ctypes.c_void_p)
 
-        rc = lib.phdm_fill(m, parr(out["ghost_x"]), out["ghost_map"].ctypes.data,
-                           out["volume"].ctypes.data, parr(out["dcom"]), out["area"].ctypes.data,
-                           parr(out["com"]), parr(out["normal"]), out["pair_i"].ctypes.data,
-                           out["pair_j"].ctypes.data, out["nbr_off"].ctypes.data,
-                           out["nbr_face"].ctypes.data)
+        def ptr(key):
+            return out[key].ctypes.data if key in out else None
+
+        rc = lib.phdm_fill(m, parr("ghost_x"), ptr("ghost_map"), ptr("volume"), parr("dcom"),
+                           ptr("area"), parr("com"), parr("normal"), ptr("pair_i"),
+                           ptr("pair_j"), ptr("nbr_off"), ptr("nbr_face"))
+        if rc == 0:
+            rc = lib.phdm_fill_topology(m, ptr("ghost_code"), ptr("ring_off"), ptr("ring_k"),
+                                        ptr("ring_code"))
         if rc != 0:
             raise RuntimeError("mesh fill failed")
-        out.pop("_keep", None)
         out["n_real"] = n
         out["n_ghost"] = ng
         return out

@@ -18,7 +18,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "phd_oracle.c")
-PINS = ["tests/test_oracle_riemann.py", "tests/test_oracle_cells.py"]
+PINS = ["tests/test_oracle_riemann.py", "tests/test_oracle_cells.py",
+        "tests/test_oracle_geometry.py"]
 
 # name: [(text in phd_oracle.c, replacement), ...]  (riemann.pyx / reconstruction.pyx lines)
 MUTATIONS = {
@@ -64,6 +65,31 @@ MUTATIONS = {
     # face velocities, mesh.pyx:470-477
     "face_velocity_sign": [("fv[k][m] = 0.5*(wx[k][i] + wx[k][j]) + factor*(x[k][j] - x[k][i]);",
                             "fv[k][m] = 0.5*(wx[k][i] + wx[k][j]) - factor*(x[k][j] - x[k][i]);")],
+    # geometry extraction, tess.cpp:313-355, tess3.cpp:452-559, boundary.pyx
+    "circumcentre_3d_operands": [("p[k] = (aa*bc[k] + bb*ca[k] + cc*ab[k])*r;",
+                                  "p[k] = (aa*bc[k] + bb*ab[k] + cc*ca[k])*r;")],
+    "circumcentre_2d_sign": [("p[1] = (bb*a[0] - aa*b[0])*r;", "p[1] = (aa*b[0] - bb*a[0])*r;")],
+    "vertex_mean_divisor": [("for (k = 0; k < 3; k++) c[k] *= 1.0/m;",
+                             "for (k = 0; k < 3; k++) c[k] *= 1.0/(m + 1);")],
+    "fan_centroid_third": [("s[k] += (c[k] + (v1[k] + v2[k])*third)*a3;",
+                            "s[k] += (c[k] + (v1[k] + v2[k])*fourth)*a3;")],
+    "face_area_half": [("area[f] = 0.5*area1;", "area[f] = area1;")],
+    "tetra_centroid_fourth": [("mo[k] += (fc[k] + (v1[k] + v2[k] + cvo[k])*fourth)*v4;",
+                               "mo[k] += (fc[k] + (v1[k] + v2[k] + cvo[k])*third)*v4;")],
+    "other_side_frame": [("mj[k] += ((fc[k] + (v1[k] + v2[k] + cvj[k])*fourth) - a[k])*v4;",
+                          "mj[k] += (fc[k] + (v1[k] + v2[k] + cvj[k])*fourth)*v4;")],
+    "other_side_apex": [("cvj[k] = a[k] - fc[k];", "cvj[k] = 0.0 - fc[k];")],
+    "volume_sixth": [("volume[i] = v*(1.0/6.0);", "volume[i] = v*(1.0/3.0);")],
+    "triangle_centroid_2d": [("ws[(size_t)f*2*W + 1 + k] = wt*(2.0*fm[k]/3.0);",
+                              "ws[(size_t)f*2*W + 1 + k] = wt*(fm[k]/3.0);")],
+    "other_side_2d": [("ws[(size_t)f*2*W + W + 1 + k] = wt*(2.0*(fm[k] - a[k])/3.0);",
+                       "ws[(size_t)f*2*W + W + 1 + k] = wt*(2.0*(fm[k] + a[k])/3.0);")],
+    "side_pick": [("(pair_i[fe] == i ? 0 : W)", "(pair_i[fe] == i ? W : 0)")],
+    "periodic_shift_digit": [("return xk + (double)(idx % 3 - 1)*(hi[a] - lo[a]);",
+                              "return xk + (double)(idx % 3 - 1)*(hi[0] - lo[0]);")],
+    "periodic_shift_base": [("for (b = 0; b < a; b++) idx /= 3;", "for (b = 0; b < a; b++) idx /= 2;")],
+    "mirror_wall": [("const double edge = ((code - 1) & 1) ? hi[a] : lo[a];",
+                     "const double edge = ((code - 1) & 1) ? lo[a] : hi[a];")],
 }
 
 
