@@ -408,6 +408,110 @@ def ensemble_timing(n_problems=4096, steps=100, seed=0, log=None, schedule=None,
     return out
 
 
+def geometry_bytes(dim, nr, ng, nf, n_ring):
+    """Minimum HBM bytes of phd_extract_geometry: every input read once and
+    every output written once.  Reads: pair_i, pair_j, ring_off (12 B per
+    face), ring entries (index + code, 5 B each), face lists (2 x 4 B per
+    face + 4 B per cell offset), generator positions (8 D B per real cell),
+    ghost map + code (5 B per ghost).  Writes: face area, centroid, normal
+    (8 (1 + 2 D) B per face), volume + centroid offset (8 (1 + D) B per cell,
+    ghosts too), ghost positions (8 D B per ghost).  The per-face workspace
+    (2 (1 + D) doubles written, half of it read back per side) is this
+    design's and counted apart."""
+    na = nr + ng
+    reads = 12 * nf + 5 * n_ring + 8 * nf + 4 * (na + 1) + 8 * dim * nr + 5 * ng
+    writes = 8 * (1 + 2 * dim) * nf + 8 * (1 + dim) * na + 8 * dim * ng
+    workspace = 8 * 2 * (1 + dim) * nf * 2
+    return {"model": reads + writes, "workspace": workspace}
+
+
+def remesh_timing(prob, reps=10):
+    """Re-meshing of the benchmark mesh, split as Mesh.build_geometry does:
+    device -> host positions, the host topology-only build (all host
+    threads), topology upload, phd_extract_geometry on the device (HIP events
+    on the library's stream; repeated on the live mesh, same values).  The
+    host builder's full build (its own geometry on the host) and its upload
+    size are measured beside it."""
+    import numpy as np
+    import torch
+    from phd.mesh import build_voronoi, extract_geometry, upload_topology
+    pc, mesh, dm = prob.particles, prob.mesh, prob.domain_manager
+    dim, nr = prob.dim, prob.n_real
+    pos = pc.carray_named_groups["position"]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x = [pc[f][:nr].cpu().numpy() for f in pos]
+    t1 = time.perf_counter()
+    bnd = dm.boundary_condition.native_code
+    topo = build_voronoi(dim, x, dm.xmin, dm.xmax, bnd, geometry=False)
+    t2 = time.perf_counter()
+    dev = pc.device
+    tdev = upload_topology(topo, dev)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    full = build_voronoi(dim, x, dm.xmin, dm.xmax, bnd, geometry=True)
+    t4 = time.perf_counter()
+    keys_full = ("ghost_x", "ghost_map", "volume", "dcom", "area", "com", "normal", "pair_i",
+                 "pair_j", "nbr_off", "nbr_face")
+    keys_topo = ("ghost_map", "ghost_code", "pair_i", "pair_j", "ring_off", "ring_k",
+                 "ring_code", "nbr_off", "nbr_face")
+
+    def nbytes(d, keys):
+        return int(sum(sum(a.nbytes for a in d[k]) if isinstance(d[k], list) else d[k].nbytes
+                       for k in keys))
+    same = all(np.array_equal(topo[k], full[k]) for k in keys_topo if k in full)
+    full_bytes = nbytes(full, keys_full)
+    del full
+    fc = mesh.faces
+    stream = torch.cuda.current_stream()
+    times = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        extract_geometry(tdev, dim, dm.xmin, dm.xmax, bnd, [pc[f] for f in pos], pc["volume"],
+                         [pc[f] for f in pc.carray_named_groups["dcom"]], fc)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ms = sorted(times)[len(times) // 2]
+    nf, ng = topo["pair_i"].size, topo["ghost_map"].size
+    B = geometry_bytes(dim, nr, ng, nf, topo["ring_k"].size)
+    return {
+        "workload": "re-mesh of the benchmark mesh: %d cells, %d ghosts, %d faces, %d ring "
+                    "entries" % (nr, ng, nf, topo["ring_k"].size),
+        "positions_d2h_ms": round((t1 - t0) * 1e3, 2),
+        "host_topology_ms": round((t2 - t1) * 1e3, 1),
+        "topology_h2d_ms": round((t3 - t2) * 1e3, 2),
+        "topology_h2d_bytes": nbytes(topo, keys_topo),
+        "host_full_build_ms": round((t4 - t3) * 1e3, 1),
+        "host_full_upload_bytes": full_bytes,
+        "topology_equal_to_full_build": bool(same),
+        "device_geometry_ms": round(ms, 4),
+        "device_geometry_algorithmic_bytes": B["model"],
+        "device_geometry_workspace_bytes": B["workspace"],
+        "device_geometry_achieved_gbs": round(B["model"] / (ms * 1e-3) / 1e9, 1),
+        "device_geometry_frac_hbm": round(B["model"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "host_threads": int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count()),
+        "_topology": topo,
+        "_x": x,
+        "_box": (list(dm.xmin), list(dm.xmax), bnd),
+    }
+
+
+def cpu_geometry_omp(x, topo, lo, hi, boundary, reps=3):
+    """The oracle's geometry extraction (OpenMP build, all host threads) on
+    the same topology: the CPU line beside phd_extract_geometry."""
+    from oracle import oracle
+    dim = len(x)
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        oracle.extract_geometry(dim, x, topo, lo, hi, boundary, omp=True)
+        t = time.perf_counter() - t0
+        best = t if best is None else min(best, t)
+    return round(best * 1e3, 1)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -611,6 +715,9 @@ def main():
         "adjoint": None,
         "ensemble": None,
     }
+    remesh = remesh_timing(prob)
+    topo_full, x_full, box = remesh.pop("_topology"), remesh.pop("_x"), remesh.pop("_box")
+    result["remesh"] = remesh
     # host copies for the CPU lines, then free the device problem
     a_full = problems.host_arrays(prob) if not args.no_cpu_baseline else None
     del L, step, prob, plan
@@ -635,6 +742,7 @@ def main():
         del st, ps
         result["cpu_baseline"] = cpu_baseline(a_s, dt_s, n_s, a_full, dt)
         result["cpu_baseline_omp"] = cpu_baseline_omp(a_full, dt, args.n)
+        result["remesh"]["cpu_geometry_omp_ms"] = cpu_geometry_omp(x_full, topo_full, *box)
     print(json.dumps(result), flush=True)
 
 

@@ -303,8 +303,8 @@ int phd_update_ghost_fields(phd_handle* h, const phd_cells* c,
  * Tess3d::extract_geometry (tess3.cpp:334-562), called by Mesh.build_geometry
  * (mesh.pyx:255-323) after the host's topology-only build
  * (phdm_build_topology, phd_mesh.h).  The topology is the host's triangulation
- * in the builder's layout: faces in owner order (pair_i real, pair_j > pair_i
- * or a ghost), ghosts n_real .. n_real + n_ghost - 1 with their image's real
+ * in the builder's layout: faces in owner order (pair_i real and non-decreasing,
+ * pair_j > pair_i or a ghost), ghosts n_real .. n_real + n_ghost - 1 with their image's real
  * index and image code, each face's ring (the generators whose bisectors
  * bound it, in polygon order) and the per-cell face lists.  All arrays are
  * device arrays. */
@@ -335,8 +335,9 @@ typedef struct phd_geometry {
 } phd_geometry;
 
 /* Bytes of the workspace phd_extract_geometry needs: 2 (1 + dim) doubles
- * per face (each side's volume term and centroid moment). */
-size_t phd_geometry_workspace_bytes(int dim, int64_t n_faces);
+ * per face (each side's volume term and centroid moment), rounded up to 256
+ * bytes, then two int32 per real cell (its run of owned faces). */
+size_t phd_geometry_workspace_bytes(int dim, int64_t n_real, int64_t n_faces);
 /* Ghost positions (the reference's image formulas, boundary.pyx:103-104,
  * 378-380), then face area, centroid and normal and cell volume and dcom,
  * with the reference's formulas (Voronoi vertices are circumcentres of the

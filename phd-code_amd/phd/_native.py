@@ -136,7 +136,7 @@ _FLUX_SIGS = {
     "phd_first_outside": ([c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_i64)], c_i32),
     "phd_reflect_fields": ([c_vp, c_vp, c_vp, c_vp], c_i32),
     "phd_update_ghost_fields": ([c_vp, c_vp, c_vp, c_vp, c_i32], c_i32),
-    "phd_geometry_workspace_bytes": ([c_i32, c_i64], ctypes.c_size_t),
+    "phd_geometry_workspace_bytes": ([c_i32, c_i64, c_i64], ctypes.c_size_t),
     "phd_extract_geometry": ([c_vp, c_vp, c_vp, c_vp, ctypes.c_size_t, ctypes.POINTER(c_i64)],
                              c_i32),
     "phd_compute_time_step_noalloc": ([c_vp, c_dbl, c_dbl, c_i32, c_vp, ctypes.POINTER(c_dbl)],

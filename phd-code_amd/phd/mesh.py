@@ -284,8 +284,8 @@ def extract_geometry(topo, dim, lo, hi, boundary, x, volume, dcom, faces=None, a
             g.com[a], g.normal[a] = ptr(com[a]), ptr(normal[a])
     g.volume = ptr(volume)
     g.area = ptr(area) if nf else None
-    nbytes = int(lib.phd_geometry_workspace_bytes(dim, nf))
-    ws = torch.empty(max(1, nbytes // 8), dtype=torch.float64, device=volume.device)
+    nbytes = int(lib.phd_geometry_workspace_bytes(dim, topo["n_real"], nf))
+    ws = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device=volume.device)
     bad = ctypes.c_int64(-1)
     rc = lib.phd_extract_geometry(h, ctypes.byref(t), ctypes.byref(g), ws.data_ptr(), nbytes,
                                   ctypes.byref(bad))

@@ -58,3 +58,18 @@ def assert_bitwise(got, want, name=""):
         i = int(bad[0])
         raise AssertionError("%s: %d of %d entries differ; first at %d: got %r want %r" % (
             name, bad.size, got.size, i, got[i], want[i]))
+
+
+def assert_time_step(dt, dt_ref, dim, name=""):
+    """The CFL time step of RiemannBase.compute_time_step: identical in 2D,
+    where the cell radius is sqrt(V / pi) (correctly rounded on both sides);
+    in 3D the radius is pow(3 V / (4 pi), 1/3) (riemann.pyx:155-158), and
+    the GPU's pow and the host libm's are each within an ulp of the true
+    power without always giving the same bits (glibc's own pow is not
+    correctly rounded on about 1 in 1300 operands), so the minimum may differ
+    by one ulp."""
+    if dim == 2:
+        assert dt == dt_ref, (name, dt, dt_ref)
+    else:
+        lo, hi = np.nextafter(dt_ref, 0.0), np.nextafter(dt_ref, np.inf)
+        assert lo <= dt <= hi, (name, dt, dt_ref)

@@ -22,7 +22,7 @@ import numpy as np
 import pytest
 import torch
 
-from parity import assert_bitwise, check_fields, rel_check
+from parity import assert_bitwise, assert_time_step, check_fields, rel_check
 
 pytestmark = pytest.mark.gpu
 
@@ -46,7 +46,7 @@ def _frozen_step_vs_oracle(p, n_steps, name):
         dt = step.time_step()
         dt_ref = oracle.time_step(dim, a["gamma"], p.riemann.cfl, p.riemann.boost, a["tag"],
                                   prim, a["volume"])
-        assert dt == dt_ref, (name, s, dt, dt_ref)
+        assert_time_step(dt, dt_ref, dim, "%s step %d" % (name, s))
         b = dict(a, cons=cons, prim=prim)
         ref = oracle.core_step(b, dt, omp=True)
         assert ref["rc"] == 0

@@ -90,6 +90,17 @@ def test_dyadic_lattice_exact(gpu, dim):
         assert np.all(got["dcom"][a] == 0.0)
 
 
+@pytest.mark.parametrize("cap", ["0", "97"])
+def test_unstaged_faces_match_oracle(gpu, monkeypatch, cap):
+    """The 3D face kernel stages each block's ring entries in LDS and falls
+    back to reading (and recomputing) them from global memory for entries
+    past its LDS capacity; PHD_GEOMETRY_STAGE_CAP lowers that capacity so
+    every face (0) or most faces (97 entries per block) take the fallback."""
+    from phd.problems import jittered_lattice
+    monkeypatch.setenv("PHD_GEOMETRY_STAGE_CAP", cap)
+    _compare(3, jittered_lattice(10, 3, seed=8), 1, "jittered 3D periodic, stage cap %s" % cap)
+
+
 def test_benchmark_mesh_matches_oracle(gpu):
     """The 3D 128^3 jittered lattice of the Sedov benchmark (2.1 M cells,
     16.6 M faces); the oracle runs its OpenMP build (same bits)."""
@@ -126,22 +137,26 @@ def test_build_geometry_device_and_host_paths(gpu):
                                atol=1e-15)
 
 
-@pytest.mark.parametrize("field,value,expect", [("ring_k", 10 ** 6, "face"),
-                                                ("pair_j", -3, "face"),
-                                                ("nbr_face", 10 ** 7, "cell"),
-                                                ("ghost_map", 10 ** 6, "cell")])
-def test_invalid_topology_is_reported(gpu, field, value, expect):
+@pytest.mark.parametrize("dim,field,value,expect", [(2, "ring_k", 10 ** 6, "face"),
+                                                    (2, "pair_j", -3, "face"),
+                                                    (2, "nbr_face", 10 ** 7, "cell"),
+                                                    (2, "ghost_map", 10 ** 6, "cell"),
+                                                    (3, "ring_k", 10 ** 6, "face"),
+                                                    (3, "ring_code", 100, "face"),
+                                                    (3, "pair_i", 10 ** 6, "face")])
+def test_invalid_topology_is_reported(gpu, dim, field, value, expect):
     """Every index is range-checked on the device before it is used: an
     out-of-range entry gives PHD_E_ARG and names the face or cell."""
     from phd.mesh import build_voronoi
     from phd.problems import jittered_lattice
-    x = jittered_lattice(10, 2, seed=1)
-    topo = build_voronoi(2, x, [0.0, 0.0], [1.0, 1.0], 0, geometry=False)
+    x = jittered_lattice(10 if dim == 2 else 5, dim, seed=1)
+    box = [0.0] * dim, [1.0] * dim
+    topo = build_voronoi(dim, x, box[0], box[1], 0, geometry=False)
     topo[field] = topo[field].copy()
     pos = topo[field].size // 2
     topo[field][pos] = value
-    print("case invalid %s[%d] = %d" % (field, pos, value), flush=True)
+    print("case invalid %dD %s[%d] = %d" % (dim, field, pos, value), flush=True)
     with pytest.raises(RuntimeError, match="invalid topology entry \\(%s" % expect):
-        _device_geometry(2, x, topo, [0.0, 0.0], [1.0, 1.0], 0)
+        _device_geometry(dim, x, topo, box[0], box[1], 0)
     from phd import _native as nat
     nat.raise_device_fault(nat.handle())   # the step error word is untouched

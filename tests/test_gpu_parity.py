@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 import torch
 
-from parity import check_fields, rel_check
+from parity import assert_time_step, check_fields, rel_check
 
 pytestmark = pytest.mark.gpu
 
@@ -54,13 +54,15 @@ def test_stage_parity(gpu, name, n):
     dt_ref = oracle.time_step(dim, a0["gamma"], p.riemann.cfl, p.riemann.boost, a0["tag"],
                               a0["prim"], a0["volume"])
     rel_check(np.array([dt]), np.array([dt_ref]), name="dt")
-    assert dt == dt_ref
+    assert_time_step(dt, dt_ref, dim, name)
 
     # generator + face velocities (mesh.pyx:363-477)
     p.mesh.assign_generator_velocities(pc, p.eos)
     w_ref = oracle.generator_velocities(dim, a0["gamma"], mesh.eta, mesh.regularize,
                                         a0["prim"], a0["dcom"], a0["volume"])
-    check_fields(_np(pc.group("w")), w_ref, "w", bitwise=True)
+    # 3D: the cell radius is pow(3V/(4 pi), 1/3) (mesh.pyx:414-417), within an
+    # ulp on both sides but not always the same bits; 2D uses sqrt
+    check_fields(_np(pc.group("w")), w_ref, "w", bitwise=dim == 2)
     p.mesh.assign_face_velocities(pc)
     a = problems.host_arrays(p)
     fv_ref = oracle.face_velocities(dim, a["x"], a["w"], a["com"], a["pair_i"], a["pair_j"])
