@@ -406,15 +406,19 @@ int phd_batch_step(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c,
                    const double* lo, const double* hi, double gamma, double cfl,
                    int boost, int gizmo, int flags);
 /* Capture one phd_batch_step (packed rows fresh, as every step leaves them)
- * into a graph on the batch's own stream.  The arrays named by the
- * arguments must stay allocated while the graph is used. */
+ * into a graph on the batch's own stream.  The captured update writes the
+ * real cells' primitive arrays only once per phd_batch_launch, after its
+ * last step (no kernel of a step reads them; the values are the same).  The
+ * arrays named by the arguments must stay allocated while the graph is
+ * used. */
 int phd_batch_capture(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c,
                       const phd_faces* f, double* rec, int32_t rec_ld, int boundary,
                       const double* lo, const double* hi, double gamma, double cfl,
                       int boost, int gizmo);
-/* n_steps graph launches, ordered after the work already on h's stream and
- * before the work enqueued there afterwards; no host synchronisation.  The
- * cells must hold a refreshed state (packed rows fresh). */
+/* n_steps graph launches, then the real cells' primitive arrays from the
+ * packed rows, ordered after the work already on h's stream and before the
+ * work enqueued there afterwards; no host synchronisation.  The cells must
+ * hold a refreshed state (packed rows fresh). */
 int phd_batch_launch(phd_handle* h, phd_batch* b, int32_t n_steps);
 /* Reset the step counter of the history and the fault marks (on h's
  * stream). */

@@ -1171,8 +1171,11 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_ghost_refresh_eos(phd_cells c, in
 // permutes within windows): the updated conserved values pass through LDS
 // and every field is then written in natural cell order, whole cache lines
 // per wave.  Ghosts are left to k_batch_ghosts, launched after this kernel
-// (their images are the updated real cells).
-template <int D, int BS = PHD_BLOCK, int XCD = 1>
+// (their images are the updated real cells).  PRIM = 0 leaves the real
+// cells' primitive arrays alone: no kernel of a batched step reads them (the
+// gradients read the packed rows), so replayed steps defer them to one
+// k_batch_prim after the last step.
+template <int D, int BS = PHD_BLOCK, int XCD = 1, int PRIM = 1>
 __global__ __launch_bounds__(BS) void k_update_eos(phd_cells c, const double* __restrict__ g,
                                                    int owner_runs, const int2* __restrict__ csr,
                                                    const int32_t* __restrict__ off,
@@ -1221,7 +1224,7 @@ __global__ __launch_bounds__(BS) void k_update_eos(phd_cells c, const double* __
 #pragma unroll
         for (int q = 0; q < NF; q++) {
             c.cons[q][i] = u[q];
-            c.prim[q][i] = p[q];
+            if (PRIM) c.prim[q][i] = p[q];
         }
         store_xp_prim<D>(xp, i, p);
     }
@@ -1249,6 +1252,18 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_batch_ghosts(phd_cells c,
 #pragma unroll
     for (int q = 0; q < NF; q++) c.prim[q][i] = p[q];
     store_xp_prim<D>(xp, i, p);
+}
+
+// primitive arrays from the packed rows' primitive part (the same values the
+// step's EOS wrote there), after steps whose update deferred them
+template <int D>
+__global__ __launch_bounds__(PHD_BLOCK) void k_batch_prim(phd_cells c, const double* __restrict__ xp)
+{
+    const long long i = xcd_index(c.n_cells);
+    if (i >= c.n_cells) return;
+    const double* r = xp + (size_t)i * XPW + XpLay<D>::P;
+#pragma unroll
+    for (int q = 0; q < D + 2; q++) c.prim[q][i] = r[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -1939,6 +1954,8 @@ struct phd_batch {
     hipEvent_t ev_in, ev_out;
     hipGraph_t graph;
     hipGraphExec_t exec;
+    phd_cells cap_cells;    // the captured step's cells and packed rows, for the
+    double* cap_xp;         // primitive arrays written after the replays
     phd_schedule sched;
     void* base;
     size_t bytes;
@@ -2069,16 +2086,23 @@ int batch_face(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c, con
 // the update fused with the real cells' epilogue, then the ghosts
 template <int D>
 int batch_update_eos(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c, int boundary,
-                     const double* lo, const double* hi, double gamma, int boost)
+                     const double* lo, const double* hi, double gamma, int boost, bool prim)
 {
     const phd_schedule& s = b->sched;
     const long long n = c->n_cells;
     bool done = false;
 #define X(BS, XC)                                                                              \
     if (!done && s.update_block == BS && s.xcd == XC) {                                        \
-        k_update_eos<D, BS, XC><<<sched_grid<BS, XC>(n), BS, 0, h->stream>>>(                  \
-            *c, p->g, p->owner_runs, p->csr, p->off, p->perm, p->sbase, p->sfi, p->cinfo,      \
-            h->d_err, gamma, p->xp, b->cell_start, b->n_problems, boost, b->dtbits, b->badp);  \
+        if (prim)                                                                              \
+            k_update_eos<D, BS, XC, 1><<<sched_grid<BS, XC>(n), BS, 0, h->stream>>>(           \
+                *c, p->g, p->owner_runs, p->csr, p->off, p->perm, p->sbase, p->sfi, p->cinfo,  \
+                h->d_err, gamma, p->xp, b->cell_start, b->n_problems, boost, b->dtbits,        \
+                b->badp);                                                                      \
+        else                                                                                   \
+            k_update_eos<D, BS, XC, 0><<<sched_grid<BS, XC>(n), BS, 0, h->stream>>>(           \
+                *c, p->g, p->owner_runs, p->csr, p->off, p->perm, p->sbase, p->sfi, p->cinfo,  \
+                h->d_err, gamma, p->xp, b->cell_start, b->n_problems, boost, b->dtbits,        \
+                b->badp);                                                                      \
         LAUNCHCHK();                                                                           \
         done = true;                                                                           \
     }
@@ -2128,6 +2152,10 @@ int batch_refresh(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c, 
 
 unsigned pblocks(int n) { return (unsigned)((n + PHD_BLOCK - 1) / PHD_BLOCK); }
 
+// batch_step flag (internal, above the public PHD_CORE_* bits): the update
+// leaves the real cells' primitive arrays to a k_batch_prim after the steps
+constexpr int BATCH_DEFER_PRIM = 1 << 16;
+
 template <int D>
 int batch_step(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c, const phd_faces* f,
                double* rec, int ld, int boundary, const double* lo, const double* hi,
@@ -2141,7 +2169,8 @@ int batch_step(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* c, con
     if (!rc) rc = batch_grad<D>(h, p, c, b->sched, rec, ld);
     if (!rc) rc = phd_update_ghost_gradients(h, p, c, boundary, lo, hi, rec, ld);
     if (!rc) rc = batch_face<D>(h, b, p, c, f, rec, ld, gamma, boost, gizmo);
-    if (!rc) rc = batch_update_eos<D>(h, b, p, c, boundary, lo, hi, gamma, boost);
+    if (!rc) rc = batch_update_eos<D>(h, b, p, c, boundary, lo, hi, gamma, boost,
+                                      !(flags & BATCH_DEFER_PRIM));
     if (rc) return rc;
     k_batch_tick<<<1, 64, 0, h->stream>>>(b->counter);
     LAUNCHCHK();
@@ -2288,8 +2317,9 @@ int phd_batch_capture(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells*
     phd_handle hc = *h;            // same error words, the batch's stream
     hc.stream = b->stream;
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    rc = c->dim == 2 ? batch_step<2>(&hc, b, p, c, f, rec, rec_ld, boundary, lo, hi, gamma, cfl, boost, gizmo, PHD_CORE_XP_FRESH)
-                     : batch_step<3>(&hc, b, p, c, f, rec, rec_ld, boundary, lo, hi, gamma, cfl, boost, gizmo, PHD_CORE_XP_FRESH);
+    const int fl = PHD_CORE_XP_FRESH | BATCH_DEFER_PRIM;
+    rc = c->dim == 2 ? batch_step<2>(&hc, b, p, c, f, rec, rec_ld, boundary, lo, hi, gamma, cfl, boost, gizmo, fl)
+                     : batch_step<3>(&hc, b, p, c, f, rec, rec_ld, boundary, lo, hi, gamma, cfl, boost, gizmo, fl);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc || e != hipSuccess || !g) {
@@ -2298,6 +2328,8 @@ int phd_batch_capture(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells*
         return rc ? rc : PHD_E_HIP;
     }
     b->graph = g;
+    b->cap_cells = *c;
+    b->cap_xp = p->xp;
     if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
         (void)hipGetLastError();
         drop_graph(b);
@@ -2312,6 +2344,13 @@ int phd_batch_launch(phd_handle* h, phd_batch* b, int32_t n_steps)
     HIPCHK(hipEventRecord(b->ev_in, h->stream));
     HIPCHK(hipStreamWaitEvent(b->stream, b->ev_in, 0));
     for (int s = 0; s < n_steps; s++) HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    if (n_steps > 0) {
+        // the captured steps deferred the real cells' primitive arrays
+        const phd_cells& cc = b->cap_cells;
+        if (cc.dim == 2) k_batch_prim<2><<<xcd_grid(cc.n_cells), PHD_BLOCK, 0, b->stream>>>(cc, b->cap_xp);
+        else k_batch_prim<3><<<xcd_grid(cc.n_cells), PHD_BLOCK, 0, b->stream>>>(cc, b->cap_xp);
+        LAUNCHCHK();
+    }
     HIPCHK(hipEventRecord(b->ev_out, b->stream));
     HIPCHK(hipStreamWaitEvent(h->stream, b->ev_out, 0));
     return 0;
