@@ -78,12 +78,17 @@ template <int D> struct FrLay {
 constexpr int GEO = 4;       // doubles per cell geometry row
 constexpr int XPW = 8;       // doubles per packed cell row
 
-// Packed cell row: [position | primitive | 0] in 3D, [primitive | position |
-// 0 0] in 2D, where the four primitives fill the row's first aligned 32-byte
-// sector, so the per-step refresh of a static mesh rewrites one whole sector.
+// Packed cell row: [position | primitive] in 3D, [primitive | position |
+// centroid] in 2D, where the four primitives fill the row's first aligned
+// 32-byte sector, so the per-step refresh of a static mesh rewrites one whole
+// sector.  The 2D centroid x + dcom is written with the plan (it is the
+// plan's cell_geo centroid) and left alone by the per-step refreshes.
 template <int D> struct XpLay {
     static constexpr int X = D == 2 ? 4 : 0;     // first position component
     static constexpr int P = D == 2 ? 0 : 3;     // first primitive
+    static constexpr int C = D == 2 ? 6 : -1;    // first centroid component (2D)
+    // double2 pairs a refresh rewrites: all but the 2D centroid pair
+    static constexpr int REFRESH_PAIRS = D == 2 ? 3 : XPW / 2;
 };
 
 struct FaceArrays { double* q[5]; };

@@ -198,7 +198,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_plan_faces(phd_cells c, phd_faces
 }
 
 template <int D>
-__global__ __launch_bounds__(PHD_BLOCK) void k_plan_cells(phd_cells c, double* geo)
+__global__ __launch_bounds__(PHD_BLOCK) void k_plan_cells(phd_cells c, double* geo, double* xp)
 {
     long long i = xcd_index(c.n_cells);
     if (i >= c.n_cells) return;
@@ -209,6 +209,8 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_plan_cells(phd_cells c, double* g
     double2* o = reinterpret_cast<double2*>(geo + (size_t)i * GEO);
     o[0] = make_double2(r[0], r[1]);
     o[1] = make_double2(r[2], r[3]);
+    if (D == 2)   // the packed row's centroid pair
+        reinterpret_cast<double2*>(xp + (size_t)i * XPW + XpLay<D>::C)[0] = make_double2(r[0], r[1]);
 }
 
 // per entry {face id, other cell | incoming bit}; the neighbor check of
@@ -351,7 +353,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_pack_xp(phd_cells c, double* xp)
     for (int n = 0; n < D + 2; n++) r[XpLay<D>::P + n] = c.prim[n][i];
     double2* o = reinterpret_cast<double2*>(xp + (size_t)i * XPW);
 #pragma unroll
-    for (int q = 0; q < XPW / 2; q++) o[q] = make_double2(r[2 * q], r[2 * q + 1]);
+    for (int q = 0; q < XpLay<D>::REFRESH_PAIRS; q++) o[q] = make_double2(r[2 * q], r[2 * q + 1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -435,6 +437,7 @@ __device__ __forceinline__ void load_row(const double* xp, int j, double* r)
 template <int D, int LIM, int BS = PHD_BLOCK, int XCD = 1>
 __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __restrict__ tag,
                                                          const double* __restrict__ geo,
+                                                         const double* __restrict__ vols,
                                                          const double* __restrict__ xp,
                                                          const int32_t* __restrict__ off,
                                                          const int32_t* __restrict__ perm,
@@ -475,12 +478,20 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
 
     double cx[D], vol, xi[D], pv[NF];
     {
-        double g[GEO];
-        ld2(geo + (size_t)i * GEO, g[0], g[1]);
-        ld2(geo + (size_t)i * GEO + 2, g[2], g[3]);
+        if (D == 2) {
+            // the packed row's centroid pair and the dense volume array: 24
+            // bytes per cell less than the 32-byte geometry row
 #pragma unroll
-        for (int k = 0; k < D; k++) cx[k] = g[k];
-        vol = g[3];
+            for (int k = 0; k < D; k++) cx[k] = own[XpLay<D>::C + k];
+            vol = vols[i];
+        } else {
+            double g[GEO];
+            ld2(geo + (size_t)i * GEO, g[0], g[1]);
+            ld2(geo + (size_t)i * GEO + 2, g[2], g[3]);
+#pragma unroll
+            for (int k = 0; k < D; k++) cx[k] = g[k];
+            vol = g[3];
+        }
 #pragma unroll
         for (int k = 0; k < D; k++) xi[k] = own[XpLay<D>::X + k];
 #pragma unroll
@@ -1157,7 +1168,7 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_ghost_refresh_eos(phd_cells c, in
             for (int n = 0; n < NF; n++) r[XpLay<D>::P + n] = p[n];
             double2* o = reinterpret_cast<double2*>(xp + (size_t)i * XPW);
 #pragma unroll
-            for (int q = 0; q < XPW / 2; q++) o[q] = make_double2(r[2 * q], r[2 * q + 1]);
+            for (int q = 0; q < XpLay<D>::REFRESH_PAIRS; q++) o[q] = make_double2(r[2 * q], r[2 * q + 1]);
         }
     }
     if (BATCH) fold_cfl<D>(real, i, u, p, vol, gamma, boost, bcs, n_problems, s_p0, dtbits, bad);
@@ -1339,7 +1350,7 @@ int launch_grad(phd_handle* h, phd_plan* p, const phd_cells* c, int limiter, dou
         LAUNCHCHK();
     }
     const unsigned nb = xcd_grid(c->n_cells);
-#define GS(L) k_grad_sell<D, L><<<nb, PHD_BLOCK, 0, h->stream>>>(c->n_cells, c->tag, p->cell_geo, p->xp, p->off, p->perm, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], rec, ld)
+#define GS(L) k_grad_sell<D, L><<<nb, PHD_BLOCK, 0, h->stream>>>(c->n_cells, c->tag, p->cell_geo, c->volume, p->xp, p->off, p->perm, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], rec, ld)
     if (limiter == PHD_LIMITER_AREPO) GS(PHD_LIMITER_AREPO);
     else if (limiter == PHD_LIMITER_TESS) GS(PHD_LIMITER_TESS);
     else return PHD_E_MODE;
@@ -1638,8 +1649,8 @@ int phd_plan_create(phd_handle* h, const phd_cells* c, const phd_faces* f, phd_p
             else k_plan_faces<3><<<xcd_grid(nf), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->face_rec);
         }
         if (na > 0) {
-            if (D == 2) k_plan_cells<2><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, p->cell_geo);
-            else k_plan_cells<3><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, p->cell_geo);
+            if (D == 2) k_plan_cells<2><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, p->cell_geo, p->xp);
+            else k_plan_cells<3><<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, p->cell_geo, p->xp);
             k_plan_csr<<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, *f, p->csr, p->d_bad);
             k_plan_runs<<<xcd_grid(na), PHD_BLOCK, 0, h->stream>>>(*c, p->csr, p->off, p->d_bad + 1);
         }
@@ -2050,7 +2061,8 @@ int batch_grad(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_schedul
 #define X(BS, XC)                                                                              \
     if (s.grad_block == BS && s.xcd == XC) {                                                   \
         k_grad_sell<D, PHD_LIMITER_AREPO, BS, XC><<<sched_grid<BS, XC>(n), BS, 0, h->stream>>>( \
-            c->n_cells, c->tag, p->cell_geo, p->xp, p->off, p->perm, p->sbase, p->sj,          \
+            c->n_cells, c->tag, p->cell_geo, c->volume, p->xp, p->off, p->perm, p->sbase,      \
+            p->sj,                                                                             \
             p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], rec, ld);                          \
         LAUNCHCHK();                                                                           \
         return 0;                                                                              \
