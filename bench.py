@@ -48,7 +48,7 @@ PROFILES = os.path.join(ROOT, "profiles")
 # bench stage -> rocprofv3 kernel name of the 3D kernel it launches
 KERNEL_NAMES = {"gradients": "k_grad_sell<3, 0, 256, 1>",
                 "face_flux": "k_face<3, 1, 1, 2, 0, 256, 1, 3>",
-                "update": "k_update_sell<3, 256, 1>", "ghost_gradients": "k_ghost_rec<3>",
+                "update": "k_update_sell<3, 256, 1, 1>", "ghost_gradients": "k_ghost_rec<3>",
                 "epilogue": "k_ghost_refresh_eos<3, 0>"}
 COPY_KERNEL = "k_pack_xp<3>"
 CORE_STAGES = ("gradients", "ghost_gradients", "face_flux", "update")

@@ -84,8 +84,9 @@ __device__ __forceinline__ int ghost_flips(const phd_cells& c, int g, int reflec
     return m;
 }
 
-// K1a: forward face states + HLLC reverse -> lambda of the predicted states
-template <int D>
+// K1a: forward face states + HLLC reverse -> lambda of the predicted states.
+// BG: boost (bit 0) and gizmo (bit 1) as template constants, as in k_face
+template <int D, int BG>
 __global__ __launch_bounds__(PHD_BLOCK) void k_adj_face_a(phd_cells c, phd_faces f,
                                                           const double* __restrict__ frec,
                                                           const double* __restrict__ rec, int ld,
@@ -95,6 +96,8 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_adj_face_a(phd_cells c, phd_faces
 {
     using Y = Lay<D>;
     constexpr int NF = Y::NF, W = frow_width<D>();
+    boost = BG & 1;
+    gizmo = (BG >> 1) & 1;
     long long m = xcd_index(f.n_faces);
     if (m >= f.n_faces) return;
     const int i = f.pair_i[m], j = f.pair_j[m];
@@ -129,8 +132,8 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_adj_face_a(phd_cells c, phd_faces
     for (int q = 0; q < NF; q++) { o[q] = lL[q]; o[NF + q] = lR[q]; }
 }
 
-// K1b: predictor and extrapolation reverse -> the face row
-template <int D>
+// K1b: predictor and extrapolation reverse -> the face row (BG as in K1a)
+template <int D, int BG>
 __global__ __launch_bounds__(PHD_BLOCK) void k_adj_face_b(phd_cells c, phd_faces f,
                                                           const double* __restrict__ frec,
                                                           const double* __restrict__ rec, int ld,
@@ -139,6 +142,8 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_adj_face_b(phd_cells c, phd_faces
 {
     using Y = Lay<D>;
     constexpr int NF = Y::NF, G = Y::G, W = frow_width<D>();
+    boost = BG & 1;
+    gizmo = (BG >> 1) & 1;
     long long m = xcd_index(f.n_faces);
     if (m >= f.n_faces) return;
     const int i = f.pair_i[m], j = f.pair_j[m];
@@ -403,10 +408,22 @@ int step_vjp(phd_handle* h, phd_plan* p, phd_adjoint* a, const phd_cells* c, con
     const FaceArrays Lo = fa5(lam_out, D + 2), Li = fa5(lam_in, D + 2);
     if (f->n_faces > 0) {
         const Gas gas = make_gas(gamma);
-        k_adj_face_a<D><<<xcd_grid(f->n_faces), PHD_BLOCK, 0, h->stream>>>(
-            *c, *f, p->face_rec, rec, ld, gas, dt, boost, gizmo, Lo, a->frow);
-        k_adj_face_b<D><<<xcd_grid(f->n_faces), PHD_BLOCK, 0, h->stream>>>(
-            *c, *f, p->face_rec, rec, ld, gas, dt, boost, gizmo, a->frow);
+        const unsigned nb = xcd_grid(f->n_faces);
+        const int bg = (boost ? 1 : 0) | (gizmo ? 2 : 0);
+#define ADJ_FACES(B)                                                                           \
+        do {                                                                                   \
+            k_adj_face_a<D, B><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, p->face_rec, rec, ld,  \
+                                                                gas, dt, boost, gizmo, Lo,     \
+                                                                a->frow);                      \
+            k_adj_face_b<D, B><<<nb, PHD_BLOCK, 0, h->stream>>>(*c, *f, p->face_rec, rec, ld,  \
+                                                                gas, dt, boost, gizmo,         \
+                                                                a->frow);                      \
+        } while (0)
+        if (bg == 0) ADJ_FACES(0);
+        else if (bg == 1) ADJ_FACES(1);
+        else if (bg == 2) ADJ_FACES(2);
+        else ADJ_FACES(3);
+#undef ADJ_FACES
     }
     k_adj_cell<D><<<xcd_grid(a->n_real), PHD_BLOCK, 0, h->stream>>>(
         *c, a->n_real, p->csr, p->off, a->gofs, a->gids, refl, l[0], l[1], l[2], u[0], u[1], u[2],

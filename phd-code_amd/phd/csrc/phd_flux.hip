@@ -893,9 +893,9 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_update(phd_cells c, const double*
 // Conserved update of the cell at sliced position t (cell i = perm[t]):
 // u = its conserved values plus its summand rows in list order.  False for a
 // ghost (nothing to write; a ghost with faces reports a fault).
-template <int D>
+template <int D, int OR>
 __device__ __forceinline__ bool update_cell(const phd_cells& c, long long t, int i,
-                                            const double* __restrict__ g, int owner_runs,
+                                            const double* __restrict__ g,
                                             const int2* __restrict__ csr,
                                             const int32_t* __restrict__ off,
                                             const int32_t* __restrict__ sbase,
@@ -911,7 +911,7 @@ __device__ __forceinline__ bool update_cell(const phd_cells& c, long long t, int
     for (int q = 0; q < NF; q++) u[q] = c.cons[q][i];
     const int deg = end - beg;
     const long long base = sbase[t / SLICE] + (t % SLICE);
-    if (owner_runs) {
+    if (OR) {
         // incoming rows (gathered) then the owned run, B rows in flight at a
         // time; applied one by one in list order
         const int2 ci = cinfo[t];
@@ -954,10 +954,11 @@ __device__ __forceinline__ bool update_cell(const phd_cells& c, long long t, int
     return true;
 }
 
-template <int D, int BS = PHD_BLOCK, int XCD = 1>
+// OR: the plan's owner_runs flag as a template constant (the kernel holds
+// only the path it takes)
+template <int D, int BS = PHD_BLOCK, int XCD = 1, int OR = 1>
 __global__ __launch_bounds__(BS) void k_update_sell(phd_cells c,
                                                            const double* __restrict__ g,
-                                                           int owner_runs,
                                                            const int2* __restrict__ csr,
                                                            const int32_t* __restrict__ off,
                                                            const int32_t* __restrict__ perm,
@@ -977,7 +978,7 @@ __global__ __launch_bounds__(BS) void k_update_sell(phd_cells c,
     const long long w0 = t - threadIdx.x;          // the window's first cell
     const int i = t < n ? perm[t] : 0;
     double u[NF];
-    const bool real = t < n && update_cell<D>(c, t, i, g, owner_runs, csr, off, sbase, sfi, cinfo, err, u);
+    const bool real = t < n && update_cell<D, OR>(c, t, i, g, csr, off, sbase, sfi, cinfo, err, u);
     if (!STAGE) {
         if (real) {
 #pragma unroll
@@ -1194,9 +1195,9 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_ghost_refresh_eos(phd_cells c, in
 // cells' primitive arrays alone: no kernel of a batched step reads them (the
 // gradients read the packed rows), so replayed steps defer them to one
 // k_batch_prim after the last step.
-template <int D, int BS = PHD_BLOCK, int XCD = 1, int PRIM = 1>
+template <int D, int BS = PHD_BLOCK, int XCD = 1, int PRIM = 1, int OR = 1>
 __global__ __launch_bounds__(BS) void k_update_eos(phd_cells c, const double* __restrict__ g,
-                                                   int owner_runs, const int2* __restrict__ csr,
+                                                   const int2* __restrict__ csr,
                                                    const int32_t* __restrict__ off,
                                                    const int32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ sbase,
@@ -1219,7 +1220,7 @@ __global__ __launch_bounds__(BS) void k_update_eos(phd_cells c, const double* __
     if (threadIdx.x == 0) s_p0 = ws < n ? problem_of((int)ws, bcs, n_problems) : 0;
     double u[NF], p[NF];
     int i = t < n ? perm[t] : 0;
-    bool real = t < n && update_cell<D>(c, t, i, g, owner_runs, csr, off, sbase, sfi, cinfo, err, u);
+    bool real = t < n && update_cell<D, OR>(c, t, i, g, csr, off, sbase, sfi, cinfo, err, u);
     if (STAGE) {
         if (real) {
 #pragma unroll
@@ -1433,9 +1434,12 @@ int launch_update(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_face
                   FaceArrays flux, double dt)
 {
     if (c->n_cells == 0) return 0;
-    if (SRC == 1)
-        k_update_sell<D><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(
-            *c, p->g, p->owner_runs, p->csr, p->off, p->perm, p->sbase, p->sfi, p->cinfo, h->d_err);
+    if (SRC == 1 && p->owner_runs)
+        k_update_sell<D, PHD_BLOCK, 1, 1><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(
+            *c, p->g, p->csr, p->off, p->perm, p->sbase, p->sfi, p->cinfo, h->d_err);
+    else if (SRC == 1)
+        k_update_sell<D, PHD_BLOCK, 1, 0><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(
+            *c, p->g, p->csr, p->off, p->perm, p->sbase, p->sfi, p->cinfo, h->d_err);
     else
         k_update<D><<<xcd_grid(c->n_cells), PHD_BLOCK, 0, h->stream>>>(
             *c, f ? f->area : nullptr, flux, p->csr, p->off, dt, h->d_err);
@@ -2129,23 +2133,22 @@ int batch_update_eos(phd_handle* h, phd_batch* b, phd_plan* p, const phd_cells* 
     const phd_schedule& s = b->sched;
     const long long n = c->n_cells;
     bool done = false;
+#define XU(BS, XC, PR, OR)                                                                     \
+    k_update_eos<D, BS, XC, PR, OR><<<sched_grid<BS, XC>(n), BS, 0, h->stream>>>(              \
+        *c, p->g, p->csr, p->off, p->perm, p->sbase, p->sfi, p->cinfo, h->d_err, gamma, p->xp,  \
+        b->cell_start, b->n_problems, boost, b->dtbits, b->badp)
 #define X(BS, XC)                                                                              \
     if (!done && s.update_block == BS && s.xcd == XC) {                                        \
-        if (prim)                                                                              \
-            k_update_eos<D, BS, XC, 1><<<sched_grid<BS, XC>(n), BS, 0, h->stream>>>(           \
-                *c, p->g, p->owner_runs, p->csr, p->off, p->perm, p->sbase, p->sfi, p->cinfo,  \
-                h->d_err, gamma, p->xp, b->cell_start, b->n_problems, boost, b->dtbits,        \
-                b->badp);                                                                      \
-        else                                                                                   \
-            k_update_eos<D, BS, XC, 0><<<sched_grid<BS, XC>(n), BS, 0, h->stream>>>(           \
-                *c, p->g, p->owner_runs, p->csr, p->off, p->perm, p->sbase, p->sfi, p->cinfo,  \
-                h->d_err, gamma, p->xp, b->cell_start, b->n_problems, boost, b->dtbits,        \
-                b->badp);                                                                      \
+        if (prim && p->owner_runs) XU(BS, XC, 1, 1);                                           \
+        else if (prim) XU(BS, XC, 1, 0);                                                       \
+        else if (p->owner_runs) XU(BS, XC, 0, 1);                                              \
+        else XU(BS, XC, 0, 0);                                                                 \
         LAUNCHCHK();                                                                           \
         done = true;                                                                           \
     }
     PHD_SCHED_CASES(X)
 #undef X
+#undef XU
     if (!done) return PHD_E_ARG;
     if (b->n_ghosts > 0) {
         double l[3], u[3];
