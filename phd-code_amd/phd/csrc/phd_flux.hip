@@ -457,6 +457,14 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
     // entries in 3D, 9/10 in 2D) are read from there instead of gathered.
     constexpr bool STAGE = BS == SORT_WINDOW;
     __shared__ double2 rows[STAGE ? SORT_WINDOW * XPW / 2 : 1];
+    // 2D AREPO: the first KC entries of each list keep their face centroid
+    // in LDS from the gradient pass for the limiter pass ([entry][thread],
+    // thread-private), so the limiter streams only the later slots again.
+    // 32 KB with the rows: four blocks still share a CU, as the 128 VGPRs
+    // allow (4096-problem step -0.8 %).  In 3D the 6 entries that would fit
+    // beside three blocks changed nothing measurable.
+    constexpr int KC = STAGE && LIM == PHD_LIMITER_AREPO && D == 2 ? 4 : 0;
+    __shared__ double2 ccache[KC > 0 ? KC * SORT_WINDOW : 1];
     const long long t = sched_index<BS, XCD>(n_cells);
     const bool live = t < n_cells;
     const int i = live ? perm[t] : 0;
@@ -518,6 +526,12 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
         for (int b = 0; b < GB; b++) {
             cur[b].j = INVALID;
             if (e0 + b < deg) cur[b] = load_slot<D>(sj, sx, sy, sz, sa, base + (long long)(e0 + b) * SLICE);
+        }
+        if (KC > 0 && e0 < KC) {
+#pragma unroll
+            for (int b = 0; b < GB; b++)
+                if (e0 + b < KC && e0 + b < deg)
+                    ccache[(e0 + b) * SORT_WINDOW + threadIdx.x] = make_double2(cur[b].com[0], cur[b].com[1]);
         }
 #pragma unroll
         for (int b = 0; b < GB; b++) {
@@ -590,6 +604,12 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
 #pragma unroll
             for (int b = 0; b < LB; b++) {
                 if (e0 + b >= deg) continue;
+                if (KC > 0 && e0 + b < KC) {
+                    const double2 v = ccache[(e0 + b) * SORT_WINDOW + threadIdx.x];
+                    relb[b][0] = v.x - cx[0];
+                    relb[b][1] = v.y - cx[1];
+                    continue;
+                }
                 const long long slot = base + (long long)(e0 + b) * SLICE;
                 relb[b][0] = __builtin_nontemporal_load(sx + slot) - cx[0];
                 relb[b][1] = __builtin_nontemporal_load(sy + slot) - cx[1];
