@@ -249,8 +249,12 @@ def raise_device_fault(h):
 
 
 def record_ld(dim):
-    """Doubles per cell-record row (include/phd_flux.h phd_cell_record_ld)."""
-    return 24 if dim == 3 else 14
+    """Doubles per cell-record row, as the library reports it
+    (include/phd_flux.h phd_cell_record_ld; loading needs no GPU)."""
+    ld = load_flux_library().phd_cell_record_ld(int(dim))
+    if ld < 0:
+        raise ValueError("no cell-record layout for dim %r" % (dim,))
+    return ld
 
 
 class Plan(object):

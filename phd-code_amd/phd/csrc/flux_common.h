@@ -57,6 +57,16 @@ struct phd_plan {
                                  // follows all its incoming faces (faces in owner order)
     int32_t* ghosts;             // third allocation: the ghost cells (tag 1), ascending
     int n_ghosts;
+    // fourth allocation: each window's halo, the distinct cells outside the
+    // window that its real cells' lists name (ascending, hoff[w] .. hoff[w+1]
+    // in halo), and per slot the row of the gradient kernel's LDS staging
+    // that holds the neighbour (sloc: window rows 0..255, then halo rows;
+    // LOC_GLOBAL when the halo overflows the staging, LOC_NONE for an
+    // invalid neighbour)
+    void* halo_base;
+    int32_t* hoff;
+    int32_t* halo;
+    uint16_t* sloc;
 };
 
 namespace {
@@ -68,6 +78,12 @@ constexpr unsigned long long INF_BITS = 0x7ff0000000000000ull;
 constexpr int INCOMING = (int)0x80000000u;
 constexpr int INVALID = 0x7fffffff;
 constexpr int FREC = 8;      // doubles per face record, at most
+constexpr uint16_t LOC_GLOBAL = 0xFFFE, LOC_NONE = 0xFFFF;   // sloc markers
+// halo rows the gradient kernel stages beside its window's 256 rows: 576 in
+// 3D (53 KB of LDS per block, three blocks per CU as its registers allow);
+// none in 2D, where a tenth of the entries lie outside the window and the
+// staging code would cost the kernel its fourth wave per SIMD
+template <int D> constexpr int halo_max() { return D == 2 ? 0 : 576; }
 
 // Face record: com(D) | area | normal(D) | |x_j - x_i|, 6 doubles in 2D and
 // 8 in 3D (rows stay 16-byte aligned for the paired loads)

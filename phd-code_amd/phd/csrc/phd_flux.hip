@@ -36,6 +36,7 @@
 // its x86-64 build except where libm pow() and the GPU's pow() (or, for bases
 // within 2^-20 of one, its binomial series) round differently.
 #include <string.h>
+#include <algorithm>
 #include <vector>
 #include "flux_common.h"
 
@@ -406,7 +407,7 @@ __device__ __forceinline__ SlotEnt<D> load_slot(const int32_t* sj, const double*
                                                 const double* sa, long long slot)
 {
     SlotEnt<D> s;
-    s.j = __builtin_nontemporal_load(sj + slot) & 0x7fffffff;
+    s.j = sj ? __builtin_nontemporal_load(sj + slot) & 0x7fffffff : INVALID;   // sj null: j unread
     s.com[0] = sx[slot];               // cached: the limiter pass reads them again
     s.com[1] = sy[slot];
     s.com[2] = D == 3 ? sz[slot] : 0.0;
@@ -447,16 +448,25 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
                                                          const double* __restrict__ sy,
                                                          const double* __restrict__ sz,
                                                          const double* __restrict__ sa,
+                                                         const int32_t* __restrict__ hoff,
+                                                         const int32_t* __restrict__ halo,
+                                                         const uint16_t* __restrict__ sloc,
                                                          double* __restrict__ rec, int ld)
 {
     using Y = Lay<D>;
     constexpr int NF = Y::NF;
     // A block of SORT_WINDOW threads owns one whole window of cells (the
     // degree sort permutes within windows): every thread stages its cell's
-    // packed row in LDS, and neighbours inside the window (about 3/4 of the
-    // entries in 3D, 9/10 in 2D) are read from there instead of gathered.
+    // packed row in LDS, and the block then gathers the window's halo (the
+    // distinct outside cells its lists name, up to HM of them) into the rows
+    // after it, all at once.  Each slot's sloc names the staged row, so the
+    // entry loop reads every neighbour from LDS: no entry waits on a
+    // per-lane HBM gather (about 1/4 of the entries in 3D, 1/10 in 2D
+    // lie outside the window; a wave would wait for its slowest lane).
     constexpr bool STAGE = BS == SORT_WINDOW;
-    __shared__ double2 rows[STAGE ? SORT_WINDOW * XPW / 2 : 1];
+    constexpr int HM = STAGE ? halo_max<D>() : 0;
+    constexpr bool HALO = HM > 0;
+    __shared__ double2 rows[STAGE ? (SORT_WINDOW + HM) * XPW / 2 : 1];
     // 2D AREPO: the first KC entries of each list keep their face centroid
     // in LDS from the gradient pass for the limiter pass ([entry][thread],
     // thread-private), so the limiter streams only the later slots again.
@@ -479,6 +489,20 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
 #pragma unroll
             for (int q = 0; q < XPW / 2; q++)
                 rows[(i - w0) * (XPW / 2) + q] = make_double2(own[2 * q], own[2 * q + 1]);
+        }
+        if (HALO && w0 < n_cells) {
+            const long long w = w0 / SORT_WINDOW;
+            const int h0 = hoff[w];
+            const int hn = min(hoff[w + 1] - h0, HM);
+            for (int k = threadIdx.x; k < hn; k += BS) {
+                const int j = halo[h0 + k];
+#pragma unroll
+                for (int q = 0; q < XPW / 2; q++) {
+                    double a, b;
+                    ld2(xp + (size_t)j * XPW + 2 * q, a, b);
+                    rows[(SORT_WINDOW + k) * (XPW / 2) + q] = make_double2(a, b);
+                }
+            }
         }
         __syncthreads();
     }
@@ -525,7 +549,18 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
 #pragma unroll
         for (int b = 0; b < GB; b++) {
             cur[b].j = INVALID;
-            if (e0 + b < deg) cur[b] = load_slot<D>(sj, sx, sy, sz, sa, base + (long long)(e0 + b) * SLICE);
+            if (e0 + b < deg) {
+                const long long slot = base + (long long)(e0 + b) * SLICE;
+                if (HALO) {
+                    // j carries the staged row; LOC_NONE marks an invalid
+                    // neighbour, LOC_GLOBAL one past the staged halo
+                    cur[b] = load_slot<D>(nullptr, sx, sy, sz, sa, slot);
+                    const int loc = __builtin_nontemporal_load(sloc + slot);
+                    cur[b].j = loc == LOC_NONE ? INVALID : loc;
+                } else {
+                    cur[b] = load_slot<D>(sj, sx, sy, sz, sa, slot);
+                }
+            }
         }
         if (KC > 0 && e0 < KC) {
 #pragma unroll
@@ -538,7 +573,17 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
             const int j = cur[b].j;
             if (j == INVALID) continue;
             const long long lj = (long long)j - w0;
-            if (STAGE && lj >= 0 && lj < SORT_WINDOW) {
+            if (HALO && j != LOC_GLOBAL) {
+#pragma unroll
+                for (int q = 0; q < XPW / 2; q++) {
+                    const double2 v = rows[j * (XPW / 2) + q];
+                    rb[b][2 * q] = v.x;
+                    rb[b][2 * q + 1] = v.y;
+                }
+            } else if (HALO) {
+                const long long slot = base + (long long)(e0 + b) * SLICE;
+                load_row(xp, sj[slot] & 0x7fffffff, rb[b]);
+            } else if (STAGE && lj >= 0 && lj < SORT_WINDOW) {
 #pragma unroll
                 for (int q = 0; q < XPW / 2; q++) {
                     const double2 v = rows[lj * (XPW / 2) + q];
@@ -1382,7 +1427,7 @@ int launch_grad(phd_handle* h, phd_plan* p, const phd_cells* c, int limiter, dou
         LAUNCHCHK();
     }
     const unsigned nb = xcd_grid(c->n_cells);
-#define GS(L) k_grad_sell<D, L><<<nb, PHD_BLOCK, 0, h->stream>>>(c->n_cells, c->tag, p->cell_geo, c->volume, p->xp, p->off, p->perm, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], rec, ld)
+#define GS(L) k_grad_sell<D, L><<<nb, PHD_BLOCK, 0, h->stream>>>(c->n_cells, c->tag, p->cell_geo, c->volume, p->xp, p->off, p->perm, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->hoff, p->halo, p->sloc, rec, ld)
     if (limiter == PHD_LIMITER_AREPO) GS(PHD_LIMITER_AREPO);
     else if (limiter == PHD_LIMITER_TESS) GS(PHD_LIMITER_TESS);
     else return PHD_E_MODE;
@@ -1552,6 +1597,86 @@ int build_slots(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces*
     return rc;
 }
 
+// Per window of SORT_WINDOW cells, the halo the gradient kernel stages in
+// LDS beside the window's own rows, and each slot's row in that staging.
+// Built on the host from the plan's lists (once per mesh).
+int build_halo(phd_handle* h, phd_plan* p, const phd_cells* c)
+{
+    const long long na = c->n_cells, ns = (na + SLICE - 1) / SLICE;
+    const long long nw = (na + SORT_WINDOW - 1) / SORT_WINDOW;
+    const long long ne = p->n_entries, nsl = p->n_slots;
+    const int HM = c->dim == 2 ? halo_max<2>() : halo_max<3>();
+    p->hoff = p->halo = nullptr;
+    p->sloc = nullptr;
+    if (HM == 0) return 0;            // no halo staging in this dimension
+    std::vector<int32_t> off((size_t)na + 1), perm((size_t)na), sb((size_t)ns + 1);
+    std::vector<int2> csr((size_t)ne);
+    std::vector<int8_t> tag((size_t)na);
+    if (na > 0) {
+        HIPCHK(hipMemcpyAsync(off.data(), p->off, 4 * (size_t)(na + 1), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(perm.data(), p->perm, 4 * (size_t)na, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(sb.data(), p->sbase, 4 * (size_t)(ns + 1), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(tag.data(), c->tag, (size_t)na, hipMemcpyDeviceToHost, h->stream));
+        if (ne > 0)
+            HIPCHK(hipMemcpyAsync(csr.data(), p->csr, sizeof(int2) * (size_t)ne, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    std::vector<int32_t> hoff((size_t)nw + 1), halo;
+    std::vector<uint16_t> sloc((size_t)nsl, LOC_NONE);
+    std::vector<int32_t> win;
+    for (long long w = 0; w < nw; w++) {
+        const long long w0 = w * SORT_WINDOW, w1 = std::min(na, w0 + SORT_WINDOW);
+        win.clear();
+        for (long long t = w0; t < w1; t++) {
+            const int i = perm[t];
+            if (tag[i] != 0) continue;
+            for (int e = off[i]; e < off[i + 1]; e++) {
+                const int j = csr[e].y & 0x7fffffff;
+                if (j != INVALID && (j < w0 || j >= w1)) win.push_back(j);
+            }
+        }
+        std::sort(win.begin(), win.end());
+        win.erase(std::unique(win.begin(), win.end()), win.end());
+        hoff[w] = (int32_t)halo.size();
+        halo.insert(halo.end(), win.begin(), win.end());
+        for (long long t = w0; t < w1; t++) {
+            const int i = perm[t];
+            if (tag[i] != 0) continue;
+            const long long base = sb[t / SLICE] + (t % SLICE);
+            for (int e = off[i]; e < off[i + 1]; e++) {
+                const int j = csr[e].y & 0x7fffffff;
+                uint16_t loc = LOC_NONE;
+                if (j != INVALID) {
+                    if (j >= w0 && j < w1) {
+                        loc = (uint16_t)(j - w0);
+                    } else {
+                        const long long k = std::lower_bound(win.begin(), win.end(), j) - win.begin();
+                        loc = k < HM ? (uint16_t)(SORT_WINDOW + k) : LOC_GLOBAL;
+                    }
+                }
+                sloc[(size_t)(base + (long long)(e - off[i]) * SLICE)] = loc;
+            }
+        }
+    }
+    hoff[(size_t)nw] = (int32_t)halo.size();
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t s_ho = al(4 * (size_t)(nw + 1)), s_ha = al(4 * halo.size() + 4);
+    const size_t s_sl = al(2 * (size_t)nsl + 2);
+    if (hipMalloc(&p->halo_base, s_ho + s_ha + s_sl) != hipSuccess) { p->halo_base = nullptr; return PHD_E_HIP; }
+    p->bytes += s_ho + s_ha + s_sl;
+    char* b = (char*)p->halo_base;
+    p->hoff = (int32_t*)b; b += s_ho;
+    p->halo = (int32_t*)b; b += s_ha;
+    p->sloc = (uint16_t*)b;
+    HIPCHK(hipMemcpyAsync(p->hoff, hoff.data(), 4 * hoff.size(), hipMemcpyHostToDevice, h->stream));
+    if (!halo.empty())
+        HIPCHK(hipMemcpyAsync(p->halo, halo.data(), 4 * halo.size(), hipMemcpyHostToDevice, h->stream));
+    if (nsl > 0)
+        HIPCHK(hipMemcpyAsync(p->sloc, sloc.data(), 2 * (size_t)nsl, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
 } // namespace
 
 // ===========================================================================
@@ -1687,6 +1812,7 @@ int phd_plan_create(phd_handle* h, const phd_cells* c, const phd_faces* f, phd_p
     p->slot_base = nullptr;
     p->ghosts = nullptr;
     p->n_ghosts = 0;
+    p->halo_base = nullptr;
     int rc = 0;
     do {
         if (hipMemcpyAsync(p->off, f->nbr_off, sizeof(int32_t) * (size_t)(na + 1), hipMemcpyDeviceToDevice, h->stream) != hipSuccess) { rc = PHD_E_HIP; break; }
@@ -1708,10 +1834,12 @@ int phd_plan_create(phd_handle* h, const phd_cells* c, const phd_faces* f, phd_p
         p->bad_cell = bad[0] == NOERR ? -1 : (long long)bad[0];
         p->owner_runs = bad[1] == NOERR;
         rc = build_slots(h, p, c, f);
+        if (!rc) rc = build_halo(h, p, c);
         if (!rc) rc = build_ghost_list(h, p, c);
     } while (0);
     if (rc) {
         if (p->ghosts) (void)hipFree(p->ghosts);
+        if (p->halo_base) (void)hipFree(p->halo_base);
         if (p->slot_base) (void)hipFree(p->slot_base);
         (void)hipFree(p->base);
         delete p;
@@ -1726,6 +1854,7 @@ int phd_plan_destroy(phd_plan* p)
     if (!p) return PHD_E_ARG;
     (void)hipDeviceSynchronize();
     if (p->ghosts) (void)hipFree(p->ghosts);
+    if (p->halo_base) (void)hipFree(p->halo_base);
     if (p->slot_base) (void)hipFree(p->slot_base);
     (void)hipFree(p->base);
     delete p;
@@ -2109,8 +2238,8 @@ int batch_grad(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_schedul
     if (s.grad_block == BS && s.xcd == XC) {                                                   \
         k_grad_sell<D, PHD_LIMITER_AREPO, BS, XC><<<sched_grid<BS, XC>(n), BS, 0, h->stream>>>( \
             c->n_cells, c->tag, p->cell_geo, c->volume, p->xp, p->off, p->perm, p->sbase,      \
-            p->sj,                                                                             \
-            p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], rec, ld);                          \
+            p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->hoff, p->halo, p->sloc,  \
+            rec, ld);                                                                          \
         LAUNCHCHK();                                                                           \
         return 0;                                                                              \
     }
