@@ -401,13 +401,13 @@ template <int D> struct SlotEnt {
     double area;
 };
 
-template <int D>
+template <int D, bool J = true>
 __device__ __forceinline__ SlotEnt<D> load_slot(const int32_t* sj, const double* sx,
                                                 const double* sy, const double* sz,
                                                 const double* sa, long long slot)
 {
     SlotEnt<D> s;
-    s.j = sj ? __builtin_nontemporal_load(sj + slot) & 0x7fffffff : INVALID;   // sj null: j unread
+    s.j = J ? __builtin_nontemporal_load(sj + slot) & 0x7fffffff : INVALID;   // !J: j unread
     s.com[0] = sx[slot];               // cached: the limiter pass reads them again
     s.com[1] = sy[slot];
     s.com[2] = D == 3 ? sz[slot] : 0.0;
@@ -554,7 +554,7 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
                 if (HALO) {
                     // j carries the staged row; LOC_NONE marks an invalid
                     // neighbour, LOC_GLOBAL one past the staged halo
-                    cur[b] = load_slot<D>(nullptr, sx, sy, sz, sa, slot);
+                    cur[b] = load_slot<D, false>(sj, sx, sy, sz, sa, slot);
                     const int loc = __builtin_nontemporal_load(sloc + slot);
                     cur[b].j = loc == LOC_NONE ? INVALID : loc;
                 } else {
