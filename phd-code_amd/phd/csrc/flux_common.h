@@ -57,14 +57,14 @@ struct phd_plan {
                                  // follows all its incoming faces (faces in owner order)
     int32_t* ghosts;             // third allocation: the ghost cells (tag 1), ascending
     int n_ghosts;
-    // fourth allocation: each window's halo, the distinct cells outside the
-    // window that its real cells' lists name (ascending, hoff[w] .. hoff[w+1]
-    // in halo), and per slot the row of the gradient kernel's LDS staging
-    // that holds the neighbour (sloc: window rows 0..255, then halo rows;
-    // LOC_GLOBAL when the halo overflows the staging, LOC_NONE for an
-    // invalid neighbour)
+    // fourth allocation (3D): each window's halo, the first halo_max of the
+    // distinct cells outside the window that its real cells' lists name
+    // (ascending, halo[w * halo_max + k] for k < hcnt[w]), and per slot the
+    // row of the gradient kernel's LDS staging that holds the neighbour
+    // (sloc: window rows 0..255, then halo rows; LOC_GLOBAL past the
+    // staging, LOC_NONE for an invalid neighbour)
     void* halo_base;
-    int32_t* hoff;
+    int32_t* hcnt;
     int32_t* halo;
     uint16_t* sloc;
 };

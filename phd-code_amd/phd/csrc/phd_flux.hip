@@ -340,6 +340,139 @@ __global__ __launch_bounds__(PHD_BLOCK) void k_plan_slots(phd_cells c, phd_faces
     cinfo[t] = make_int2(n_in, own0);
 }
 
+// Exclusive prefix sum of one int per thread over a block of SORT_WINDOW
+// threads; total receives the block's sum.  part: SORT_WINDOW ints of LDS.
+__device__ __forceinline__ int block_scan_excl(int v, int* part, int& total)
+{
+    const int tid = threadIdx.x;
+    part[tid] = v;
+    __syncthreads();
+    for (int d = 1; d < SORT_WINDOW; d <<= 1) {
+        const int a = tid >= d ? part[tid - d] : 0;
+        __syncthreads();
+        part[tid] += a;
+        __syncthreads();
+    }
+    total = part[SORT_WINDOW - 1];
+    const int ex = part[tid] - v;
+    __syncthreads();
+    return ex;
+}
+
+// The 3D gradient kernel's halo of window w (cells [w0, w1) = positions
+// [w0, w1), since the degree sort permutes within windows), one block per
+// window: the list entries of its real cells that name a cell outside the
+// window are gathered into LDS, sorted (bitonic) and made distinct; the
+// first hm are the halo, halo[w*hm + k] in ascending order, hcnt[w] of them.
+// Each list slot then gets its staging row: j - w0 inside the window,
+// SORT_WINDOW + k for the k-th halo cell, LOC_GLOBAL for a distinct cell past
+// hm, LOC_NONE for an invalid neighbour (padding slots keep the LOC_NONE the
+// host fills in).  A window whose lists name more than HALO_CAP outside
+// entries keeps no halo: all of them are LOC_GLOBAL (read from HBM).
+constexpr int HALO_CAP = 4096;
+
+__global__ __launch_bounds__(SORT_WINDOW) void k_plan_halo(long long na,
+                                                           const int8_t* __restrict__ tag,
+                                                           const int32_t* __restrict__ off,
+                                                           const int32_t* __restrict__ perm,
+                                                           const int32_t* __restrict__ sbase,
+                                                           const int32_t* __restrict__ sj,
+                                                           int hm, int32_t* __restrict__ hcnt,
+                                                           int32_t* __restrict__ halo,
+                                                           uint16_t* __restrict__ sloc)
+{
+    __shared__ int32_t key[HALO_CAP];
+    __shared__ int part[SORT_WINDOW];
+    const long long w = blockIdx.x;
+    const long long w0 = w * SORT_WINDOW, w1 = min(na, w0 + SORT_WINDOW);
+    const long long t = w0 + threadIdx.x;
+    int deg = 0;
+    long long base = 0;
+    if (t < w1) {
+        const int i = perm[t];
+        if (tag[i] == 0) {
+            deg = off[i + 1] - off[i];
+            base = sbase[t / SLICE] + (t % SLICE);
+        }
+    }
+    auto outside = [&](int j) { return j != INVALID && (j < w0 || j >= w1); };
+    int mine = 0;
+    for (int r = 0; r < deg; r++) mine += outside(sj[base + (long long)r * SLICE] & 0x7fffffff);
+    int n = 0;
+    int at = block_scan_excl(mine, part, n);
+    const bool fit = n <= HALO_CAP;               // block-uniform
+    int u = 0;
+    if (fit) {
+        for (int r = 0; r < deg && mine > 0; r++) {
+            const int j = sj[base + (long long)r * SLICE] & 0x7fffffff;
+            if (outside(j)) key[at++] = j;
+        }
+        int np = 1;
+        while (np < n) np <<= 1;
+        for (int k = n + threadIdx.x; k < np; k += SORT_WINDOW) key[k] = INT_MAX;
+        __syncthreads();
+        for (int k = 2; k <= np; k <<= 1) {
+            for (int d = k >> 1; d > 0; d >>= 1) {
+                for (int a = threadIdx.x; a < np; a += SORT_WINDOW) {
+                    const int b = a ^ d;
+                    if (b > a) {
+                        const int x = key[a], y = key[b];
+                        if ((x > y) == ((a & k) == 0)) { key[a] = y; key[b] = x; }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // distinct values of key[0, n): each thread reads a run of at most
+        // 16, counts the first of each value, and after the block's reads
+        // and scan writes them packed from the front
+        constexpr int CH = HALO_CAP / SORT_WINDOW;
+        const int c0 = threadIdx.x * CH, c1 = min(n, c0 + CH);
+        int v[CH];
+        unsigned first = 0;
+        int nu = 0;
+        int last = (c0 > 0 && c0 < n) ? key[c0 - 1] : -1;
+#pragma unroll
+        for (int q = 0; q < CH; q++) {
+            v[q] = 0;
+            if (c0 + q < c1) {
+                v[q] = key[c0 + q];
+                if (v[q] != last) { first |= 1u << q; nu++; }
+                last = v[q];
+            }
+        }
+        int o = block_scan_excl(nu, part, u);     // its barriers follow every read above
+#pragma unroll
+        for (int q = 0; q < CH; q++)
+            if ((first >> q) & 1u) key[o++] = v[q];
+        __syncthreads();
+    }
+    const int kept = min(u, hm);
+    if (threadIdx.x == 0) hcnt[w] = kept;
+    for (int k = threadIdx.x; k < kept; k += SORT_WINDOW) halo[w * hm + k] = key[k];
+    for (int r = 0; r < deg; r++) {
+        const long long slot = base + (long long)r * SLICE;
+        const int j = sj[slot] & 0x7fffffff;
+        uint16_t loc = LOC_NONE;
+        if (j != INVALID) {
+            if (j >= w0 && j < w1) {
+                loc = (uint16_t)(j - w0);
+            } else if (!fit) {
+                loc = LOC_GLOBAL;
+            } else {
+                int lo = 0, hi = u;                   // first key >= j (present)
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (key[mid] < j) lo = mid + 1;
+                    else hi = mid;
+                }
+                loc = lo < hm ? (uint16_t)(SORT_WINDOW + lo) : LOC_GLOBAL;
+            }
+        }
+        sloc[slot] = loc;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // packed [position | primitive] rows for the neighbour gathers
 template <int D>
@@ -448,7 +581,7 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
                                                          const double* __restrict__ sy,
                                                          const double* __restrict__ sz,
                                                          const double* __restrict__ sa,
-                                                         const int32_t* __restrict__ hoff,
+                                                         const int32_t* __restrict__ hcnt,
                                                          const int32_t* __restrict__ halo,
                                                          const uint16_t* __restrict__ sloc,
                                                          double* __restrict__ rec, int ld)
@@ -492,10 +625,9 @@ __global__ __launch_bounds__(BS) void k_grad_sell(int n_cells, const int8_t* __r
         }
         if (HALO && w0 < n_cells) {
             const long long w = w0 / SORT_WINDOW;
-            const int h0 = hoff[w];
-            const int hn = min(hoff[w + 1] - h0, HM);
+            const int hn = hcnt[w];                // at most HM
             for (int k = threadIdx.x; k < hn; k += BS) {
-                const int j = halo[h0 + k];
+                const int j = halo[w * HM + k];
 #pragma unroll
                 for (int q = 0; q < XPW / 2; q++) {
                     double a, b;
@@ -1427,7 +1559,7 @@ int launch_grad(phd_handle* h, phd_plan* p, const phd_cells* c, int limiter, dou
         LAUNCHCHK();
     }
     const unsigned nb = xcd_grid(c->n_cells);
-#define GS(L) k_grad_sell<D, L><<<nb, PHD_BLOCK, 0, h->stream>>>(c->n_cells, c->tag, p->cell_geo, c->volume, p->xp, p->off, p->perm, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->hoff, p->halo, p->sloc, rec, ld)
+#define GS(L) k_grad_sell<D, L><<<nb, PHD_BLOCK, 0, h->stream>>>(c->n_cells, c->tag, p->cell_geo, c->volume, p->xp, p->off, p->perm, p->sbase, p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->hcnt, p->halo, p->sloc, rec, ld)
     if (limiter == PHD_LIMITER_AREPO) GS(PHD_LIMITER_AREPO);
     else if (limiter == PHD_LIMITER_TESS) GS(PHD_LIMITER_TESS);
     else return PHD_E_MODE;
@@ -1597,83 +1729,33 @@ int build_slots(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces*
     return rc;
 }
 
-// Per window of SORT_WINDOW cells, the halo the gradient kernel stages in
-// LDS beside the window's own rows, and each slot's row in that staging.
-// Built on the host from the plan's lists (once per mesh).
+// Per window of SORT_WINDOW cells, the halo the 3D gradient kernel stages
+// in LDS beside the window's own rows (at most halo_max rows, fixed stride
+// per window), and each slot's row in that staging: k_plan_halo, one block
+// per window, after the sliced lists are built.
 int build_halo(phd_handle* h, phd_plan* p, const phd_cells* c)
 {
-    const long long na = c->n_cells, ns = (na + SLICE - 1) / SLICE;
-    const long long nw = (na + SORT_WINDOW - 1) / SORT_WINDOW;
-    const long long ne = p->n_entries, nsl = p->n_slots;
+    const long long na = c->n_cells, nw = (na + SORT_WINDOW - 1) / SORT_WINDOW;
+    const long long nsl = p->n_slots;
     const int HM = c->dim == 2 ? halo_max<2>() : halo_max<3>();
-    p->hoff = p->halo = nullptr;
+    p->hcnt = p->halo = nullptr;
     p->sloc = nullptr;
     if (HM == 0) return 0;            // no halo staging in this dimension
-    std::vector<int32_t> off((size_t)na + 1), perm((size_t)na), sb((size_t)ns + 1);
-    std::vector<int2> csr((size_t)ne);
-    std::vector<int8_t> tag((size_t)na);
-    if (na > 0) {
-        HIPCHK(hipMemcpyAsync(off.data(), p->off, 4 * (size_t)(na + 1), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipMemcpyAsync(perm.data(), p->perm, 4 * (size_t)na, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipMemcpyAsync(sb.data(), p->sbase, 4 * (size_t)(ns + 1), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipMemcpyAsync(tag.data(), c->tag, (size_t)na, hipMemcpyDeviceToHost, h->stream));
-        if (ne > 0)
-            HIPCHK(hipMemcpyAsync(csr.data(), p->csr, sizeof(int2) * (size_t)ne, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
-    }
-    std::vector<int32_t> hoff((size_t)nw + 1), halo;
-    std::vector<uint16_t> sloc((size_t)nsl, LOC_NONE);
-    std::vector<int32_t> win;
-    for (long long w = 0; w < nw; w++) {
-        const long long w0 = w * SORT_WINDOW, w1 = std::min(na, w0 + SORT_WINDOW);
-        win.clear();
-        for (long long t = w0; t < w1; t++) {
-            const int i = perm[t];
-            if (tag[i] != 0) continue;
-            for (int e = off[i]; e < off[i + 1]; e++) {
-                const int j = csr[e].y & 0x7fffffff;
-                if (j != INVALID && (j < w0 || j >= w1)) win.push_back(j);
-            }
-        }
-        std::sort(win.begin(), win.end());
-        win.erase(std::unique(win.begin(), win.end()), win.end());
-        hoff[w] = (int32_t)halo.size();
-        halo.insert(halo.end(), win.begin(), win.end());
-        for (long long t = w0; t < w1; t++) {
-            const int i = perm[t];
-            if (tag[i] != 0) continue;
-            const long long base = sb[t / SLICE] + (t % SLICE);
-            for (int e = off[i]; e < off[i + 1]; e++) {
-                const int j = csr[e].y & 0x7fffffff;
-                uint16_t loc = LOC_NONE;
-                if (j != INVALID) {
-                    if (j >= w0 && j < w1) {
-                        loc = (uint16_t)(j - w0);
-                    } else {
-                        const long long k = std::lower_bound(win.begin(), win.end(), j) - win.begin();
-                        loc = k < HM ? (uint16_t)(SORT_WINDOW + k) : LOC_GLOBAL;
-                    }
-                }
-                sloc[(size_t)(base + (long long)(e - off[i]) * SLICE)] = loc;
-            }
-        }
-    }
-    hoff[(size_t)nw] = (int32_t)halo.size();
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t s_ho = al(4 * (size_t)(nw + 1)), s_ha = al(4 * halo.size() + 4);
+    const size_t s_hc = al(4 * (size_t)nw + 4), s_ha = al(4 * (size_t)nw * HM + 4);
     const size_t s_sl = al(2 * (size_t)nsl + 2);
-    if (hipMalloc(&p->halo_base, s_ho + s_ha + s_sl) != hipSuccess) { p->halo_base = nullptr; return PHD_E_HIP; }
-    p->bytes += s_ho + s_ha + s_sl;
+    if (hipMalloc(&p->halo_base, s_hc + s_ha + s_sl) != hipSuccess) { p->halo_base = nullptr; return PHD_E_HIP; }
+    p->bytes += s_hc + s_ha + s_sl;
     char* b = (char*)p->halo_base;
-    p->hoff = (int32_t*)b; b += s_ho;
+    p->hcnt = (int32_t*)b; b += s_hc;
     p->halo = (int32_t*)b; b += s_ha;
     p->sloc = (uint16_t*)b;
-    HIPCHK(hipMemcpyAsync(p->hoff, hoff.data(), 4 * hoff.size(), hipMemcpyHostToDevice, h->stream));
-    if (!halo.empty())
-        HIPCHK(hipMemcpyAsync(p->halo, halo.data(), 4 * halo.size(), hipMemcpyHostToDevice, h->stream));
-    if (nsl > 0)
-        HIPCHK(hipMemcpyAsync(p->sloc, sloc.data(), 2 * (size_t)nsl, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    if (nsl > 0) HIPCHK(hipMemsetAsync(p->sloc, 0xFF, 2 * (size_t)nsl, h->stream));   // LOC_NONE
+    if (nw > 0) {
+        k_plan_halo<<<(unsigned)nw, SORT_WINDOW, 0, h->stream>>>(na, c->tag, p->off, p->perm, p->sbase,
+                                                                 p->sj, HM, p->hcnt, p->halo, p->sloc);
+        LAUNCHCHK();
+    }
     return 0;
 }
 
@@ -2238,7 +2320,7 @@ int batch_grad(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_schedul
     if (s.grad_block == BS && s.xcd == XC) {                                                   \
         k_grad_sell<D, PHD_LIMITER_AREPO, BS, XC><<<sched_grid<BS, XC>(n), BS, 0, h->stream>>>( \
             c->n_cells, c->tag, p->cell_geo, c->volume, p->xp, p->off, p->perm, p->sbase,      \
-            p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->hoff, p->halo, p->sloc,  \
+            p->sj, p->sgeo[0], p->sgeo[1], p->sgeo[2], p->sgeo[3], p->hcnt, p->halo, p->sloc,  \
             rec, ld);                                                                          \
         LAUNCHCHK();                                                                           \
         return 0;                                                                              \
