@@ -9,7 +9,10 @@
     field away from limiter switches (cells whose one-sided differences at
     eps, 2 eps and 3 eps do not scale as a smooth function's are skipped),
     to 1e-6 relative, at 24^2 x 8 steps and at
-    256^2 x 50 steps with a checkpoint every 10.
+    256^2 x 50 steps with a checkpoint every 10;
+  - the rollout's gradient contracted with random directions against the
+    forward-mode derivative of the oracle's rollout (dual numbers) to 1e-10,
+    from 16^2 / 6^3 up to the timed configuration, KH 256^2 x 50 steps.
 """
 import numpy as np
 import pytest
@@ -221,7 +224,10 @@ def test_rollout_matches_plain_reverse_sweep(gpu):
 
 @pytest.mark.parametrize("name,n,pressure,n_steps,every", [("kh_2d", 16, None, 6, 2),
                                                           ("sod_2d", 16, None, 5, 2),
-                                                          ("sedov_3d", 6, 1.0, 4, 2)])
+                                                          ("sedov_3d", 6, 1.0, 4, 2),
+                                                          ("kh_2d", 64, None, 10, 5),
+                                                          ("sedov_3d", 12, 1.0, 4, 2),
+                                                          ("kh_2d", 256, None, 50, 10)])
 def test_rollout_against_forward_mode_ad_of_the_oracle(gpu, name, n, pressure, n_steps, every):
     """The taped reference: the oracle differentiated in forward mode
     (oracle/oracle_dual.cpp) gives the exact directional derivative of the
