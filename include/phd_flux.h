@@ -129,8 +129,9 @@ int phd_get_error(phd_handle* h, int32_t* code, int64_t* index);
 /* A plan is the device-side acceleration structure of one frozen mesh: packed
  * face records (centroid, area, normal, generator distance), cell centroids
  * and volumes, per-cell face lists as {face, other cell, incoming bit}, the
- * list of ghost cells (the tags at creation), and the per-step scratch of the
- * fused kernels.  It is built from the mesh
+ * list of ghost cells (the tags at creation), in 3D each 256-cell window's
+ * halo of outside neighbours, and the per-step scratch of the fused kernels.
+ * It is built on the device (6 ms at 2.1 M cells) from the mesh
  * arrays the reference's Mesh.build_geometry produces (mesh.pyx:255-323) and
  * must be recreated whenever the mesh is rebuilt.  The only allocating and
  * synchronising calls of the library besides the handle's. */
