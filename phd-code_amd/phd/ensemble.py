@@ -1,9 +1,9 @@
-"""Batched ensembles of independent 2D Riemann problems on the GPU.
+"""Batched ensembles of independent 2D (or 3D) Riemann problems on the GPU.
 
 Each problem is a static-mesh run (StaticMeshMUSCLHancock.evolve_timestep,
-integrate.py:349-378, with generator and face velocities zero) of a 2D
-Riemann problem in the reflective unit box: left state for x < 0.5, right
-state otherwise, on its own jittered-lattice Voronoi mesh.  The ensemble is
+integrate.py:349-378, with generator and face velocities zero) of a
+Riemann problem in the reflective unit square (cube): left state for
+x < 0.5, right state otherwise, on its own jittered-lattice Voronoi mesh.  The ensemble is
 held as one union mesh (libphd_flux.so phd_batch_*): problem p owns cells
 [cell_start[p], cell_start[p+1]) -- its reals, then its ghosts -- and a
 contiguous run of faces, with global indices.  One step of every problem is
@@ -17,9 +17,9 @@ plan), over views into the same field arrays, for the comparison and the
 parity checks.
 
 Seeded draws (numpy default_rng(seed)): cells per problem log-uniform in
-[n_min, n_max] (the lattice side is the rounded square root), left/right
-density and pressure log-uniform in [0.1, 10], left/right x-velocity uniform
-in [-1, 1], y-velocity 0; gamma 1.4; HLLC in the lab frame (boost off, so
+[n_min, n_max] (the lattice side is the rounded square or cube root),
+left/right density and pressure log-uniform in [0.1, 10], left/right
+x-velocity uniform in [-1, 1], the other components 0; gamma 1.4; HLLC in the lab frame (boost off, so
 the CFL bound includes |v|), PieceWiseLinear with the AREPO and GIZMO
 limiters, CFL 0.3.
 """
@@ -39,28 +39,32 @@ GAMMA = 1.4
 CFL = 0.3
 BOOST = 0
 GIZMO = 1
-LO = (0.0, 0.0)
-HI = (1.0, 1.0)
+LO = (0.0, 0.0, 0.0)
+HI = (1.0, 1.0, 1.0)
 
 
-def draw_problems(n_problems, seed=0, n_min=1000, n_max=64000):
-    """Seeded problem parameters (dict of numpy arrays, one entry per problem)."""
+def draw_problems(n_problems, seed=0, n_min=1000, n_max=64000, dim=2):
+    """Seeded problem parameters (dict of numpy arrays, one entry per
+    problem, and the dimension)."""
+    if dim not in (2, 3):
+        raise ValueError("dim must be 2 or 3")
     rng = np.random.default_rng(seed)
     cells = np.exp(rng.uniform(math.log(n_min), math.log(n_max), n_problems))
-    side = np.maximum(2, np.rint(np.sqrt(cells))).astype(np.int64)
+    root = np.sqrt(cells) if dim == 2 else np.cbrt(cells)
+    side = np.maximum(2, np.rint(root)).astype(np.int64)
 
     def logu(n):
         return np.exp(rng.uniform(math.log(0.1), math.log(10.0), n))
     return {"side": side, "rho_l": logu(n_problems), "rho_r": logu(n_problems),
             "p_l": logu(n_problems), "p_r": logu(n_problems),
             "v_l": rng.uniform(-1.0, 1.0, n_problems), "v_r": rng.uniform(-1.0, 1.0, n_problems),
-            "mesh_seed": rng.integers(0, 2**31 - 1, n_problems)}
+            "mesh_seed": rng.integers(0, 2**31 - 1, n_problems), "dim": dim}
 
 
 def _mesh(args):
-    side, seed = args
-    x = jittered_lattice(int(side), 2, int(seed))
-    return x, build_voronoi(2, x, LO, HI, nat.BOUNDARY_REFLECTIVE, 1)
+    side, seed, dim = args
+    x = jittered_lattice(int(side), dim, int(seed))
+    return x, build_voronoi(dim, x, LO[:dim], HI[:dim], nat.BOUNDARY_REFLECTIVE, 1)
 
 
 class HostEnsemble(object):
@@ -70,9 +74,10 @@ class HostEnsemble(object):
     def __init__(self, spec, n_workers=16):
         self.spec = spec
         P = len(spec["side"])
-        self.n_problems = P
+        D = int(spec.get("dim", 2))
+        self.n_problems, self.dim = P, D
         with ThreadPoolExecutor(max_workers=max(1, n_workers)) as ex:
-            parts = list(ex.map(_mesh, zip(spec["side"], spec["mesh_seed"])))
+            parts = list(ex.map(_mesh, zip(spec["side"], spec["mesh_seed"], [D] * P)))
         n_real = np.array([x[0].size for x, _ in parts], dtype=np.int64)
         n_ghost = np.array([g["ghost_map"].size for _, g in parts], dtype=np.int64)
         n_face = np.array([g["area"].size for _, g in parts], dtype=np.int64)
@@ -95,7 +100,7 @@ class HostEnsemble(object):
             return np.concatenate([fn(p, x, g) for p, (x, g) in enumerate(parts)]).astype(dtype)
 
         self.x = [cat(lambda p, x, g, k=k: np.concatenate([x[k], g["ghost_x"][k]]), np.float64)
-                  for k in range(2)]
+                  for k in range(D)]
         self.tag = cat(lambda p, x, g: np.concatenate(
             [np.zeros(n_real[p]), np.ones(n_ghost[p])]), np.int8)
         self.type = cat(lambda p, x, g: np.concatenate(
@@ -103,10 +108,10 @@ class HostEnsemble(object):
         self.map_local = cat(lambda p, x, g: np.concatenate(
             [np.arange(n_real[p]), g["ghost_map"]]), np.int32)
         self.volume = cat(lambda p, x, g: g["volume"], np.float64)
-        self.dcom = [cat(lambda p, x, g, k=k: g["dcom"][k], np.float64) for k in range(2)]
+        self.dcom = [cat(lambda p, x, g, k=k: g["dcom"][k], np.float64) for k in range(D)]
         self.area = cat(lambda p, x, g: g["area"], np.float64)
-        self.com = [cat(lambda p, x, g, k=k: g["com"][k], np.float64) for k in range(2)]
-        self.normal = [cat(lambda p, x, g, k=k: g["normal"][k], np.float64) for k in range(2)]
+        self.com = [cat(lambda p, x, g, k=k: g["com"][k], np.float64) for k in range(D)]
+        self.normal = [cat(lambda p, x, g, k=k: g["normal"][k], np.float64) for k in range(D)]
         self.pair_i_local = cat(lambda p, x, g: g["pair_i"], np.int32)
         self.pair_j_local = cat(lambda p, x, g: g["pair_j"], np.int32)
         self.nbr_face_local = cat(lambda p, x, g: g["nbr_face"], np.int32)
@@ -130,15 +135,13 @@ class HostEnsemble(object):
         self.density = np.where(left, spec["rho_l"][pid], spec["rho_r"][pid])
         self.pressure = np.where(left, spec["p_l"][pid], spec["p_r"][pid])
         self.vx = np.where(left, spec["v_l"][pid], spec["v_r"][pid])
-        self.vy = np.zeros(na)
+        self.v = [self.vx] + [np.zeros(na) for _ in range(D - 1)]
+        self.vy = self.v[1]
 
 
 class Ensemble(object):
     """The union mesh on the device, its plan and batch object; per-problem
     views on demand."""
-
-    FIELDS_PRIM = ("density", "velocity-x", "velocity-y", "pressure")
-    FIELDS_CONS = ("mass", "momentum-x", "momentum-y", "energy")
 
     def __init__(self, host, device=None, hist_steps=0):
         dev = torch.device("cuda", torch.cuda.current_device()) if device is None else device
@@ -149,30 +152,31 @@ class Ensemble(object):
             return torch.as_tensor(np.ascontiguousarray(a), device=dev, dtype=dtype)
         H = host
         na, nf = H.n_cells, H.n_faces
+        D = self.dim = H.dim
         self.n_cells, self.n_faces, self.n_problems = na, nf, H.n_problems
         self.x = [t(a) for a in H.x]
         self.tag, self.type = t(H.tag), t(H.type)
         self.map, self.map_local = t(H.map), t(H.map_local)
         self.volume = t(H.volume)
         self.dcom = [t(a) for a in H.dcom]
-        self.prim = [torch.zeros(na, dtype=torch.float64, device=dev) for _ in range(4)]
-        self.cons = [torch.zeros(na, dtype=torch.float64, device=dev) for _ in range(4)]
-        self.w = [torch.zeros(na, dtype=torch.float64, device=dev) for _ in range(2)]
+        self.prim = [torch.zeros(na, dtype=torch.float64, device=dev) for _ in range(D + 2)]
+        self.cons = [torch.zeros(na, dtype=torch.float64, device=dev) for _ in range(D + 2)]
+        self.w = [torch.zeros(na, dtype=torch.float64, device=dev) for _ in range(D)]
         self.area = t(H.area)
         self.com = [t(a) for a in H.com]
         self.normal = [t(a) for a in H.normal]
-        self.fvel = [torch.zeros(nf, dtype=torch.float64, device=dev) for _ in range(2)]
+        self.fvel = [torch.zeros(nf, dtype=torch.float64, device=dev) for _ in range(D)]
         self.pair_i, self.pair_j = t(H.pair_i), t(H.pair_j)
         self.nbr_off, self.nbr_face = t(H.nbr_off), t(H.nbr_face)
         self.pair_i_local, self.pair_j_local = t(H.pair_i_local), t(H.pair_j_local)
         self.nbr_off_local, self.nbr_face_local = t(H.nbr_off_local), t(H.nbr_face_local)
-        self.ld = nat.record_ld(2)
+        self.ld = nat.record_ld(D)
         self.rec = torch.zeros((na, self.ld), dtype=torch.float64, device=dev)
         self.cells = self._cells(0, na, self.map)
         self.faces = self._faces(0, nf, self.pair_i, self.pair_j, self.nbr_off, self.nbr_face)
         self.plan = nat.Plan(self.cells, self.faces)
-        self.lo = (ctypes.c_double * 3)(LO[0], LO[1], 0.0)
-        self.hi = (ctypes.c_double * 3)(HI[0], HI[1], 0.0)
+        self.lo = (ctypes.c_double * 3)(*[LO[k] if k < D else 0.0 for k in range(3)])
+        self.hi = (ctypes.c_double * 3)(*[HI[k] if k < D else 0.0 for k in range(3)])
         cs = (ctypes.c_int32 * (H.n_problems + 1))(*[int(v) for v in H.cell_start])
         b = ctypes.c_void_p()
         nat.check(self.lib.phd_batch_create(self.h, self.plan.ptr, ctypes.byref(self.cells),
@@ -188,17 +192,17 @@ class Ensemble(object):
     # -- views -------------------------------------------------------------------
     def _cells(self, c0, n, mapt):
         c = nat.Cells()
-        c.dim, c.n_cells = 2, n
+        c.dim, c.n_cells = self.dim, n
         e = 8 * c0
         c.tag = self.tag.data_ptr() + c0
         c.type = self.type.data_ptr() + c0
         c.map = mapt.data_ptr() + 4 * c0
-        for k in range(2):
+        for k in range(self.dim):
             c.x[k] = self.x[k].data_ptr() + e
             c.dcom[k] = self.dcom[k].data_ptr() + e
             c.w[k] = self.w[k].data_ptr() + e
         c.volume = self.volume.data_ptr() + e
-        for q in range(4):
+        for q in range(self.dim + 2):
             c.prim[q] = self.prim[q].data_ptr() + e
             c.cons[q] = self.cons[q].data_ptr() + e
         return c
@@ -208,7 +212,7 @@ class Ensemble(object):
         f.n_faces = n
         e = 8 * f0
         f.area = self.area.data_ptr() + e
-        for k in range(2):
+        for k in range(self.dim):
             f.com[k] = self.com[k].data_ptr() + e
             f.normal[k] = self.normal[k].data_ptr() + e
             f.velocity[k] = self.fvel[k].data_ptr() + e
@@ -239,7 +243,7 @@ class Ensemble(object):
     def reset(self):
         """Initial primitive state, conserved fields from it, ghosts refreshed."""
         H = self.host
-        for q, a in enumerate((H.density, H.vx, H.vy, H.pressure)):
+        for q, a in enumerate([H.density] + H.v + [H.pressure]):
             self.prim[q].copy_(torch.as_tensor(a, device=self.device))
         nat.check(self.lib.phd_conservative_from_primitive(self.h, ctypes.byref(self.cells), GAMMA),
                   "phd_conservative_from_primitive")

@@ -1,11 +1,13 @@
 """Union-mesh assembly of a batched ensemble (host only): global indices,
 owner order, problem isolation, face lists and ghost maps."""
 import numpy as np
+import pytest
 
 
-def _host(n=8, seed=3):
+def _host(n=8, seed=3, dim=2):
     from phd import ensemble as E
-    return E.HostEnsemble(E.draw_problems(n, seed=seed, n_min=100, n_max=900), n_workers=4)
+    return E.HostEnsemble(E.draw_problems(n, seed=seed, n_min=100, n_max=900, dim=dim),
+                          n_workers=4)
 
 
 def test_draws_are_seeded_and_in_range():
@@ -18,10 +20,14 @@ def test_draws_are_seeded_and_in_range():
     for k in ("rho_l", "rho_r", "p_l", "p_r"):
         assert a[k].min() >= 0.1 and a[k].max() <= 10.0
     assert np.abs(a["v_l"]).max() <= 1.0 and np.abs(a["v_r"]).max() <= 1.0
+    c3 = E.draw_problems(64, seed=7, dim=3)
+    assert np.array_equal(c3["rho_l"], a["rho_l"])            # same draws, cube-root sides
+    assert (c3["side"] ** 3).min() >= 729 and (c3["side"] ** 3).max() <= 68921
 
 
-def test_union_layout():
-    H = _host()
+@pytest.mark.parametrize("dim", [2, 3])
+def test_union_layout(dim):
+    H = _host(dim=dim)
     cs, fs = H.cell_start, H.face_start
     assert cs[0] == 0 and cs[-1] == H.n_cells and fs[-1] == H.n_faces
     # each block: reals then ghosts
@@ -39,8 +45,9 @@ def test_union_layout():
     assert np.array_equal(H.pair_i - np.repeat(cs[:-1], H.n_face), H.pair_i_local)
 
 
-def test_face_lists_and_ghost_maps():
-    H = _host()
+@pytest.mark.parametrize("dim", [2, 3])
+def test_face_lists_and_ghost_maps(dim):
+    H = _host(dim=dim)
     assert H.nbr_off[-1] == H.nbr_face.size and np.all(np.diff(H.nbr_off) >= 0)
     for c in range(H.n_cells):
         fl = H.nbr_face[H.nbr_off[c]:H.nbr_off[c + 1]]
@@ -61,12 +68,13 @@ def test_face_lists_and_ghost_maps():
         assert np.array_equal(lo - lo[0], H.nbr_off[cs[p]:cs[p + 1] + 1] - H.nbr_off[cs[p]])
 
 
-def test_initial_state_is_the_riemann_split():
-    H = _host()
+@pytest.mark.parametrize("dim", [2, 3])
+def test_initial_state_is_the_riemann_split(dim):
+    H = _host(dim=dim)
     s = H.spec
     pid = np.repeat(np.arange(H.n_problems), H.cell_start[1:] - H.cell_start[:-1])
     left = H.x[0] < 0.5
     assert np.array_equal(H.density[left], s["rho_l"][pid][left])
     assert np.array_equal(H.pressure[~left], s["p_r"][pid][~left])
     assert np.array_equal(H.vx[left], s["v_l"][pid][left])
-    assert not H.vy.any()
+    assert len(H.x) == len(H.v) == dim and not any(v.any() for v in H.v[1:])

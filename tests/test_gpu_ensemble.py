@@ -1,8 +1,9 @@
 """GPU checks of the batched ensemble engine (phd_batch_*): the batched
 step reproduces, bit for bit, every problem stepped alone through the
 per-problem entry points (phd_compute_time_step, phd_core_step,
-phd_ghost_refresh_eos) with its own time step; graph replays reproduce eager
-launches; launch schedules do not change results."""
+phd_ghost_refresh_eos) with its own time step, for 2D and 3D problems;
+graph replays reproduce eager launches; launch schedules do not change
+results."""
 import numpy as np
 import pytest
 import torch
@@ -10,10 +11,11 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
-def _ens(n=6, seed=11, hist=16):
+def _ens(n=6, seed=11, hist=16, dim=2):
     from phd import ensemble as E
-    print("ensemble n=%d seed=%d" % (n, seed), flush=True)
-    H = E.HostEnsemble(E.draw_problems(n, seed=seed, n_min=100, n_max=900), n_workers=4)
+    print("ensemble n=%d seed=%d dim=%d" % (n, seed, dim), flush=True)
+    H = E.HostEnsemble(E.draw_problems(n, seed=seed, n_min=100, n_max=900, dim=dim),
+                       n_workers=4)
     ens = E.Ensemble(H, hist_steps=hist)
     ens.reset()
     return ens
@@ -23,8 +25,9 @@ def _same(a, b):
     return all(torch.equal(x, y) for x, y in zip(a, b))
 
 
-def test_batched_time_steps_match_per_problem(gpu):
-    ens = _ens()
+@pytest.mark.parametrize("dim", [2, 3])
+def test_batched_time_steps_match_per_problem(gpu, dim):
+    ens = _ens(dim=dim)
     ens.reset_history()
     ens.step_eager()
     dt, k, hist, faults = ens.read(history=True)
@@ -34,8 +37,9 @@ def test_batched_time_steps_match_per_problem(gpu):
     assert np.array_equal(hist[0], ref), (hist[0], ref)
 
 
-def test_batched_steps_match_per_problem_steps(gpu):
-    ens = _ens()
+@pytest.mark.parametrize("dim", [2, 3])
+def test_batched_steps_match_per_problem_steps(gpu, dim):
+    ens = _ens(dim=dim)
     n_steps = 4
     ens.reset_history()
     for _ in range(n_steps):
@@ -60,8 +64,9 @@ def test_batched_steps_match_per_problem_steps(gpu):
     # same bits)
 
 
-def test_graph_replays_match_eager_steps(gpu):
-    ens = _ens()
+@pytest.mark.parametrize("dim", [2, 3])
+def test_graph_replays_match_eager_steps(gpu, dim):
+    ens = _ens(dim=dim)
     s0 = ens.state()
     for _ in range(3):
         ens.step_eager()
@@ -84,10 +89,11 @@ def test_graph_replays_match_eager_steps(gpu):
     ens.check()
 
 
+@pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("sched", [(64, 64, 64, 0), (128, 256, 128, 1), (256, 128, 64, 0),
                                    (64, 128, 256, 1)])
-def test_schedules_give_identical_results(gpu, sched):
-    ens = _ens()
+def test_schedules_give_identical_results(gpu, sched, dim):
+    ens = _ens(dim=dim)
     s0 = ens.state()
     ens.run(3)
     ref = ens.state()
