@@ -143,9 +143,10 @@ class Ensemble(object):
     """The union mesh on the device, its plan and batch object; per-problem
     views on demand."""
 
-    def __init__(self, host, device=None, hist_steps=0):
+    def __init__(self, host, device=None, hist_steps=0, boost=BOOST, gizmo=GIZMO):
         dev = torch.device("cuda", torch.cuda.current_device()) if device is None else device
         self.host, self.device = host, dev
+        self.boost, self.gizmo = int(boost), int(gizmo)
         self.lib, self.h = nat.flux(), nat.handle()
 
         def t(a, dtype=None):
@@ -255,7 +256,7 @@ class Ensemble(object):
         nat.check(self.lib.phd_batch_refresh(self.h, self.batch, self.plan.ptr,
                                              ctypes.byref(self.cells), nat.BOUNDARY_REFLECTIVE,
                                              ctypes.cast(self.lo, ctypes.c_void_p),
-                                             ctypes.cast(self.hi, ctypes.c_void_p), GAMMA, BOOST),
+                                             ctypes.cast(self.hi, ctypes.c_void_p), GAMMA, self.boost),
                   "phd_batch_refresh")
 
     def state(self):
@@ -283,7 +284,7 @@ class Ensemble(object):
         return (self.h, self.batch, self.plan.ptr, ctypes.byref(self.cells),
                 ctypes.byref(self.faces), self.rec.data_ptr(), self.ld, nat.BOUNDARY_REFLECTIVE,
                 ctypes.cast(self.lo, ctypes.c_void_p), ctypes.cast(self.hi, ctypes.c_void_p),
-                GAMMA, CFL, BOOST, GIZMO)
+                GAMMA, CFL, self.boost, self.gizmo)
 
     def step_eager(self):
         """One batched step by direct launches (the packed rows are fresh
@@ -343,7 +344,7 @@ class Ensemble(object):
     def time_step_problem(self, p):
         c, f, plan, rec = self.problem_views()[p]
         dt = ctypes.c_double()
-        nat.check(self.lib.phd_compute_time_step(self.h, ctypes.byref(c), GAMMA, CFL, BOOST,
+        nat.check(self.lib.phd_compute_time_step(self.h, ctypes.byref(c), GAMMA, CFL, self.boost,
                                                  ctypes.byref(dt)), "phd_compute_time_step")
         return dt.value
 
@@ -360,8 +361,8 @@ class Ensemble(object):
                 continue
             cr, fr = ctypes.byref(c), ctypes.byref(f)
             rc = lib.phd_core_step(h, plan.ptr, cr, fr, rec, self.ld, nat.LIMITER_AREPO,
-                                   nat.BOUNDARY_REFLECTIVE, lo, hi, GAMMA, float(dts[p]), BOOST,
-                                   GIZMO, nat.SOLVER_HLLC, None, flags)
+                                   nat.BOUNDARY_REFLECTIVE, lo, hi, GAMMA, float(dts[p]),
+                                   self.boost, self.gizmo, nat.SOLVER_HLLC, None, flags)
             if rc:
                 nat.check(rc, "phd_core_step")
             rc = lib.phd_ghost_refresh_eos(h, plan.ptr, cr, nat.BOUNDARY_REFLECTIVE, lo, hi, GAMMA)

@@ -11,12 +11,12 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
-def _ens(n=6, seed=11, hist=16, dim=2):
+def _ens(n=6, seed=11, hist=16, dim=2, **kw):
     from phd import ensemble as E
-    print("ensemble n=%d seed=%d dim=%d" % (n, seed, dim), flush=True)
+    print("ensemble n=%d seed=%d dim=%d %s" % (n, seed, dim, kw), flush=True)
     H = E.HostEnsemble(E.draw_problems(n, seed=seed, n_min=100, n_max=900, dim=dim),
                        n_workers=4)
-    ens = E.Ensemble(H, hist_steps=hist)
+    ens = E.Ensemble(H, hist_steps=hist, **kw)
     ens.reset()
     return ens
 
@@ -37,9 +37,13 @@ def test_batched_time_steps_match_per_problem(gpu, dim):
     assert np.array_equal(hist[0], ref), (hist[0], ref)
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_batched_steps_match_per_problem_steps(gpu, dim):
-    ens = _ens(dim=dim)
+@pytest.mark.parametrize("dim,boost,gizmo", [(2, 0, 1), (3, 0, 1), (2, 1, 1), (2, 0, 0),
+                                             (2, 1, 0), (3, 1, 0)])
+def test_batched_steps_match_per_problem_steps(gpu, dim, boost, gizmo):
+    """Every boost/gizmo instance of the batched face kernel (boost: the
+    face frame, which with the static meshes' zero face velocities is the
+    lab frame, and a CFL bound without |v|)."""
+    ens = _ens(dim=dim, boost=boost, gizmo=gizmo)
     n_steps = 4
     ens.reset_history()
     for _ in range(n_steps):
