@@ -21,7 +21,7 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
-def _stepper(name, n, seed=0, noise=0.02, vnoise=None, pressure=None):
+def _stepper(name, n, seed=0, noise=0.02, vnoise=None, pressure=None, boost=None, gizmo=None):
     """A static-mesh stepper on `name`, its primitive fields perturbed by
     seeded noise: the set-ups are piecewise uniform, and at exactly equal
     neighbour values the GIZMO branch test (pi < pj) sits on its switch,
@@ -31,6 +31,10 @@ def _stepper(name, n, seed=0, noise=0.02, vnoise=None, pressure=None):
     print("case %s n=%d" % (name, n), flush=True)
     p = problems.PROBLEMS[name](n=n, seed=seed)
     st = StaticStepper(p)
+    if boost is not None:
+        st.boost = int(boost)          # face frame (zero face velocities) / lab frame
+    if gizmo is not None:
+        st.gizmo = int(gizmo)
     pc = p.particles
     rng = np.random.default_rng(seed + 100)
     nr = int(st.real.sum())
@@ -222,13 +226,16 @@ def test_rollout_matches_plain_reverse_sweep(gpu):
         assert torch.equal(gr, ref[0]) and torch.equal(gv, ref[1]), every
 
 
-@pytest.mark.parametrize("name,n,pressure,n_steps,every", [("kh_2d", 16, None, 6, 2),
-                                                          ("sod_2d", 16, None, 5, 2),
-                                                          ("sedov_3d", 6, 1.0, 4, 2),
-                                                          ("kh_2d", 64, None, 10, 5),
-                                                          ("sedov_3d", 12, 1.0, 4, 2),
-                                                          ("kh_2d", 256, None, 50, 10)])
-def test_rollout_against_forward_mode_ad_of_the_oracle(gpu, name, n, pressure, n_steps, every):
+@pytest.mark.parametrize("name,n,pressure,n_steps,every,bg", [
+    ("kh_2d", 16, None, 6, 2, None), ("sod_2d", 16, None, 5, 2, None),
+    ("sedov_3d", 6, 1.0, 4, 2, None), ("kh_2d", 64, None, 10, 5, None),
+    ("sedov_3d", 12, 1.0, 4, 2, None), ("kh_2d", 256, None, 50, 10, None),
+    # the other boost / gizmo instances of the adjoint face kernels
+    ("kh_2d", 16, None, 6, 2, (0, 0)), ("kh_2d", 16, None, 6, 2, (0, 1)),
+    ("kh_2d", 16, None, 6, 2, (1, 0)), ("sedov_3d", 6, 1.0, 4, 2, (0, 0)),
+    ("sedov_3d", 6, 1.0, 4, 2, (0, 1)), ("sedov_3d", 6, 1.0, 4, 2, (1, 0))])
+def test_rollout_against_forward_mode_ad_of_the_oracle(gpu, name, n, pressure, n_steps, every,
+                                                       bg):
     """The taped reference: the oracle differentiated in forward mode
     (oracle/oracle_dual.cpp) gives the exact directional derivative of the
     same rollout for the branch pattern of the state, with no truncation
@@ -238,7 +245,9 @@ def test_rollout_against_forward_mode_ad_of_the_oracle(gpu, name, n, pressure, n
     from oracle import oracle
     from phd import problems
     from phd.adjoint import forward_dts, rollout
-    st = _stepper(name, n, pressure=pressure)
+    boost, gizmo = bg if bg is not None else (None, None)
+    st = _stepper(name, n, pressure=pressure, boost=boost, gizmo=gizmo)
+    print("boost %d gizmo %d" % (st.boost, st.gizmo), flush=True)
     real = st.real
     pc = st.p.particles
     rho0 = st.prim_init[0].clone()

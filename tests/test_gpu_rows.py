@@ -56,19 +56,21 @@ def _exact_scales(a, dt):
 @pytest.mark.parametrize("limiter", [0, 1])
 @pytest.mark.parametrize("solver", [0, 1, 2])
 @pytest.mark.parametrize("gizmo", [True, False])
-def test_core_step_variants(gpu, name, n, limiter, solver, gizmo):
+@pytest.mark.parametrize("boost", [True, False])
+def test_core_step_variants(gpu, name, n, limiter, solver, gizmo, boost):
     from phd import problems
     from oracle import oracle
 
     p = _problem(name, n)
     step = problems.FrozenStep(p)
     step.limiter, step.solver, step.gizmo = limiter, solver, int(gizmo)
+    step.boost = int(boost)
     step.prologue()
     # a quarter of the CFL step: the 8^3 blast (32 hot cells of 512) can
     # drive HLL's energy negative at the full step, in the reference too
     dt = 0.25 * step.time_step()
     a = problems.host_arrays(p)
-    ref = oracle.core_step(a, dt, solver=solver, boost=True, gizmo=gizmo, limiter=limiter)
+    ref = oracle.core_step(a, dt, solver=solver, boost=boost, gizmo=gizmo, limiter=limiter)
     step.run(dt)
     if ref["rc"]:
         from phd import _native as nat
@@ -88,7 +90,8 @@ def test_core_step_variants(gpu, name, n, limiter, solver, gizmo):
                  bitwise=exact_bits, scales=csc)
     g = p.recon.grad.rows().cpu().numpy()
     check_fields([g[:, q] for q in range(len(ref["grad"]))], ref["grad"], "grad", bitwise=True)
-    print("limiter", limiter, "solver", solver, "gizmo", gizmo, "flux bitwise fraction", fr)
+    print("limiter", limiter, "solver", solver, "gizmo", gizmo, "boost", boost,
+          "flux bitwise fraction", fr)
 
 
 @pytest.mark.parametrize("name,n", [("sod_2d", 20), ("kh_2d", 16), ("sedov_3d", 8)])
