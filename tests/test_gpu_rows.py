@@ -8,7 +8,8 @@ API, against the oracle on the same inputs:
     boundary.pyx:309-348, equation_state.pyx:81-117);
   - fault semantics: the first failing face of the reference's ascending
     loop and its message (mesh.pyx:546-551), on the API update and on the
-    fused step;
+    fused step; the first cell listing a face it is not part of
+    (reconstruction.pyx:646-652);
   - a whole moving-mesh Sod run (the reference's own CPU-runnable
     configuration, shortened) against the exact Riemann solution, with mass
     and energy conserved to round-off.
@@ -193,6 +194,41 @@ def test_core_step_fault_matches_oracle(gpu):
         step.check()
     assert str(e.value) == nat.FAULT_MESSAGES[ref["rc"]]
     assert e.value.index == ref["bad"]
+
+
+@pytest.mark.parametrize("name,n", [("sod_2d", 12), ("sedov_3d", 6)])
+def test_incorrect_neighbors_matches_oracle(gpu, name, n):
+    """Two real cells list a face they are not part of: the step reports the
+    reference's "ERROR: incorrect neighbors!" (reconstruction.pyx:646-652)
+    at the first such cell in the reference's loop order, as the oracle
+    does."""
+    from phd import problems
+    from phd import _native as nat
+    from oracle import oracle
+
+    p = _problem(name, n)
+    mesh = p.mesh
+    off = mesh.neighbors.offsets.cpu().numpy()
+    lst = mesh.neighbors.faces.cpu().numpy().copy()
+    pi = mesh.faces["pair-i"].cpu().numpy()
+    pj = mesh.faces["pair-j"].cpu().numpy()
+    for k in (3 * off.size // 5, off.size // 4):     # the later cell planted first
+        far = int(np.nonzero((pi != k) & (pj != k))[0][0])
+        lst[off[k]] = far
+    mesh.neighbors.faces.copy_(torch.as_tensor(lst, dtype=mesh.neighbors.faces.dtype))
+    mesh.invalidate_plan()
+    first = off.size // 4
+    step = problems.FrozenStep(p)
+    step.prologue()
+    dt = step.time_step()
+    a = problems.host_arrays(p)
+    ref = oracle.core_step(a, dt)
+    assert ref["rc"] == 1 and ref["bad"] == first, (ref["rc"], ref["bad"])
+    step.run(dt)
+    with pytest.raises(RuntimeError) as e:
+        step.check()
+    assert str(e.value) == nat.FAULT_MESSAGES[1] == "ERROR: incorrect neighbors!"
+    assert e.value.index == first
 
 
 def _exact_sod(x, t, gamma=1.4, L=(1.0, 0.0, 1.0), R=(0.125, 0.0, 0.1), x0=0.5):
