@@ -95,7 +95,8 @@ def test_graph_replays_match_eager_steps(gpu, dim):
 
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("sched", [(64, 64, 64, 0), (128, 256, 128, 1), (256, 128, 64, 0),
-                                   (64, 128, 256, 1)])
+                                   (64, 128, 256, 1), (128, 64, 128, 0), (256, 256, 256, 0),
+                                   (64, 64, 64, 1)])
 def test_schedules_give_identical_results(gpu, sched, dim):
     ens = _ens(dim=dim)
     s0 = ens.state()
