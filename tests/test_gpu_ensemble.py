@@ -107,6 +107,13 @@ def test_schedules_give_identical_results(gpu, sched, dim):
     ens.run(3)
     torch.cuda.synchronize()
     assert _same(ens.state(), ref), sched
+    # direct launches with the same schedule (the update writing the
+    # primitive arrays every step)
+    ens.restart(s0)
+    for _ in range(3):
+        ens.step_eager()
+    torch.cuda.synchronize()
+    assert _same(ens.state(), ref), sched
 
 
 def test_a_failed_problem_is_frozen_and_the_others_go_on(gpu):

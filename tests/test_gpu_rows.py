@@ -91,17 +91,36 @@ def test_core_step_variants(gpu, name, n, limiter, solver, gizmo, boost):
                  bitwise=exact_bits, scales=csc)
     g = p.recon.grad.rows().cpu().numpy()
     check_fields([g[:, q] for q in range(len(ref["grad"]))], ref["grad"], "grad", bitwise=True)
+    # the same step with the summands alone (no flux arrays written) gives
+    # the same conserved fields bit for bit
+    q2 = _problem(name, n)
+    s2 = problems.FrozenStep(q2)
+    s2.limiter, s2.solver, s2.gizmo, s2.boost = limiter, solver, int(gizmo), int(boost)
+    s2.prologue()
+    s2.run(dt, write_flux=False)
+    s2.check()
+    for x, y in zip(_np(q2.particles.group("conservative")), _np(p.particles.group("conservative"))):
+        assert np.array_equal(x[real], y[real])
     print("limiter", limiter, "solver", solver, "gizmo", gizmo, "boost", boost,
           "flux bitwise fraction", fr)
 
 
+@pytest.mark.parametrize("solver,boost", [("hllc", True), ("hllc", False), ("hll", True),
+                                          ("hll", False), ("exact", True)])
 @pytest.mark.parametrize("name,n", [("sod_2d", 20), ("kh_2d", 16), ("sedov_3d", 8)])
-def test_piecewise_constant(gpu, name, n):
+def test_piecewise_constant(gpu, name, n, solver, boost):
+    """PieceWiseConstant states and the fused constant-order flux kernel for
+    each solver and frame (Exact forces the face frame, riemann.pyx:695-698);
+    floors per face (parity.flux_scales): HLL and HLLC reach pow in the
+    two-rarefaction estimate, Exact in its iteration."""
     from phd import problems
+    from phd import _native as nat
     from phd.reconstruction import PieceWiseConstant
     from oracle import oracle
 
     p = _problem(name, n)
+    p.riemann.solver = {"hll": nat.SOLVER_HLL, "hllc": nat.SOLVER_HLLC,
+                        "exact": nat.SOLVER_EXACT}[solver]
     pc, mesh = p.particles, p.mesh
     p.mesh.assign_generator_velocities(pc, p.eos)
     p.mesh.assign_face_velocities(pc)
@@ -110,16 +129,17 @@ def test_piecewise_constant(gpu, name, n):
     pwc.add_fields(pc)
     pwc.initialize()
     pwc.compute_gradients(pc, mesh, p.domain_manager)
-    pwc.compute_states(pc, mesh, p.eos.get_gamma(), p.domain_manager, 0.01, True)
-    pl, pr = oracle.states_constant(a, boost=True)
-    check_fields(_np(pwc.left_states.group("primitive")), pl, "left")
-    check_fields(_np(pwc.right_states.group("primitive")), pr, "right")
+    pwc.compute_states(pc, mesh, p.eos.get_gamma(), p.domain_manager, 0.01, boost)
+    pl, pr = oracle.states_constant(a, boost=boost)
+    check_fields(_np(pwc.left_states.group("primitive")), pl, "left", bitwise=True)
+    check_fields(_np(pwc.right_states.group("primitive")), pr, "right", bitwise=True)
     # fused path: states rebuilt inside the flux kernel
-    pwc.compute_states(pc, mesh, p.eos.get_gamma(), p.domain_manager, 0.01, True)
+    pwc.compute_states(pc, mesh, p.eos.get_gamma(), p.domain_manager, 0.01, boost)
     p.riemann.compute_fluxes(pc, mesh, pwc, p.eos)
-    F, rc, _ = oracle.riemann("hllc", a["dim"], a["gamma"], True, pl, pr, a["normal"], a["fv"])
+    F, rc, _ = oracle.riemann(solver, a["dim"], a["gamma"], boost, pl, pr, a["normal"], a["fv"])
     assert rc == 0
-    check_fields(_np(p.riemann.fluxes.group("conservative")), F, "flux")
+    check_fields(_np(p.riemann.fluxes.group("conservative")), F, "flux",
+                 scales=flux_scales(pl, pr, a["gamma"], a["fv"]))
 
 
 @pytest.mark.parametrize("name,n", [("sod_2d", 20), ("kh_2d", 16), ("sedov_3d", 8)])
