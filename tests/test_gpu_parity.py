@@ -108,6 +108,48 @@ def test_stage_parity(gpu, name, n):
                  bitwise=True)
 
 
+@pytest.mark.parametrize("solver,boost,gizmo", [("hll", True, True), ("hll", False, False),
+                                                ("hllc", False, True), ("hllc", True, False),
+                                                ("exact", True, True), ("exact", True, False)])
+@pytest.mark.parametrize("name,n", CASES)
+def test_fused_linear_fluxes(gpu, name, n, solver, boost, gizmo):
+    """PieceWiseLinear states deferred into the flux kernel (riemann.pyx:
+    74-98) for each solver, frame and GIZMO setting, against the oracle's
+    states and solver: HLL and HLLC bit for bit on these meshes, the exact
+    solver (pow inside its iteration) to its per-face floor."""
+    from phd import problems
+    from phd import _native as nat
+    from oracle import oracle
+
+    p = _problem(name, n)
+    pc, mesh = p.particles, p.mesh
+    p.mesh.assign_generator_velocities(pc, p.eos)
+    p.mesh.assign_face_velocities(pc)
+    recon = p.recon
+    recon.gizmo_limiter = bool(gizmo)
+    p.riemann.solver = {"hll": nat.SOLVER_HLL, "hllc": nat.SOLVER_HLLC,
+                        "exact": nat.SOLVER_EXACT}[solver]
+    half = 0.5 * p.riemann.compute_time_step(pc, p.eos)
+    recon.compute_gradients(pc, mesh, p.domain_manager)
+    a = problems.host_arrays(p)
+    g_ref, rc, _ = oracle.compute_gradients(a, limiter=0)
+    assert rc == 0
+    oracle.ghost_gradients(a, g_ref)
+    pl, pr = oracle.add_spatial(a, g_ref, boost=boost, gizmo=gizmo)
+    oracle.add_temporal(a, g_ref, half, pl, pr, boost=boost)
+    F_ref, rc, _ = oracle.riemann(solver, a["dim"], a["gamma"], boost, pl, pr, a["normal"],
+                                  a["fv"])
+    assert rc == 0
+    recon.compute_states(pc, mesh, p.eos.get_gamma(), p.domain_manager, half, boost)
+    p.riemann.compute_fluxes(pc, mesh, recon, p.eos)
+    got = _np(p.riemann.fluxes.group("conservative"))
+    if solver == "exact":
+        fr = check_fields(got, F_ref, "fused flux", scales=flux_scales(pl, pr, a["gamma"], a["fv"]))
+    else:
+        fr = check_fields(got, F_ref, "fused flux", bitwise=True)
+    print(solver, boost, gizmo, "flux bitwise fraction", fr)
+
+
 @pytest.mark.parametrize("name,n", CASES)
 def test_core_step_parity(gpu, name, n):
     from phd import problems
