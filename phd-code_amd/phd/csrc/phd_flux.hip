@@ -1624,8 +1624,12 @@ int launch_face(phd_handle* h, phd_plan* p, const phd_cells* c, const phd_faces*
     if (f->n_faces == 0) return 0;
     if (order == PHD_ORDER_LINEAR)
         return launch_face_o<D, PHD_ORDER_LINEAR, OUT>(h, p, c, f, rec, ld, gamma, dt_half, dt_full, boost, gizmo, solver, flux);
-    if (order == PHD_ORDER_CONSTANT)
-        return launch_face_o<D, PHD_ORDER_CONSTANT, OUT>(h, p, c, f, rec, ld, gamma, dt_half, dt_full, boost, gizmo, solver, flux);
+    // constant order only through the flux entry point (the fused step
+    // reconstructs linearly), so only its output mode is compiled
+    if constexpr (OUT == 1) {
+        if (order == PHD_ORDER_CONSTANT)
+            return launch_face_o<D, PHD_ORDER_CONSTANT, OUT>(h, p, c, f, rec, ld, gamma, dt_half, dt_full, boost, gizmo, solver, flux);
+    }
     return PHD_E_MODE;
 }
 
