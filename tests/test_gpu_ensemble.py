@@ -180,3 +180,57 @@ def test_ragged_and_single_problem_batches(gpu, sides):
     ens.reset()
     _steps_match(ens, 3)
     ens.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_batch_with_faces_not_numbered_by_owner(gpu, dim):
+    """A one-problem batch whose faces are renumbered by a seeded permutation
+    (every per-face array moved, every cell's list remapped and sorted
+    ascending), so the plan finds no owner runs and the fused update walks
+    each list through its face ids: one batched step against the oracle's
+    step on the same renumbered arrays (time step, core step, ghost refresh
+    and EOS; the reference accepts any face numbering, mesh.pyx:479-551).
+    (A comparison with the builder's numbering would not do: summing in
+    another order changes the round-off, which can flip a branch of the
+    wave-speed estimate.)"""
+    from oracle import oracle
+    from phd import ensemble as E
+    from parity import assert_time_step, check_fields
+    H = E.HostEnsemble(E.draw_problems(1, seed=5, n_min=300, n_max=900, dim=dim), n_workers=1)
+    perm = np.random.default_rng(9).permutation(H.n_faces)   # old id -> new id
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(H.n_faces)
+    H.area, H.pair_i, H.pair_j = H.area[inv], H.pair_i[inv], H.pair_j[inv]
+    H.com = [x[inv] for x in H.com]
+    H.normal = [x[inv] for x in H.normal]
+    new = perm[H.nbr_face.astype(np.int64)]
+    for c in range(H.n_cells):
+        new[H.nbr_off[c]:H.nbr_off[c + 1]] = np.sort(new[H.nbr_off[c]:H.nbr_off[c + 1]])
+    H.nbr_face = new.astype(np.int32)
+    ens = E.Ensemble(H, hist_steps=4)
+    print("one-problem batch, faces renumbered, dim %d, %d cells, %d faces"
+          % (dim, H.n_cells, H.n_faces), flush=True)
+    ens.reset()
+    ens.reset_history()
+    before = [x.cpu().numpy() for x in ens.cons + ens.prim]
+    ens.step_eager()
+    ens.check()
+    _, k, hist, faults = ens.read(history=True)
+    assert k == 1 and not faults.any()
+    nf_ = dim + 2
+    a = {"dim": dim, "n": H.n_cells, "nf": H.n_faces, "tag": H.tag, "type": H.type,
+         "map": H.map, "x": H.x, "dcom": H.dcom, "volume": H.volume,
+         "cons": before[:nf_], "prim": before[nf_:], "area": H.area, "com": H.com,
+         "normal": H.normal, "fv": [np.zeros(H.n_faces) for _ in range(dim)],
+         "pair_i": H.pair_i, "pair_j": H.pair_j, "nbr_off": H.nbr_off, "nbr_face": H.nbr_face,
+         "lo": list(E.LO[:dim]), "hi": list(E.HI[:dim]), "boundary": 0, "gamma": E.GAMMA}
+    dt = float(hist[0][0])
+    dt_ref = oracle.time_step(dim, E.GAMMA, E.CFL, E.BOOST, a["tag"], a["prim"], a["volume"])
+    assert_time_step(dt, dt_ref, dim, "batch dt")
+    ref = oracle.core_step(a, dt, solver=1, boost=bool(E.BOOST), gizmo=True)
+    assert ref["rc"] == 0
+    cons, prim = oracle.ghost_refresh_eos(a, ref["cons"])
+    after = [x.cpu().numpy() for x in ens.cons + ens.prim]
+    check_fields(after[:nf_], cons, "renumbered batch cons")
+    check_fields(after[nf_:], prim, "renumbered batch prim")
+    ens.close()
